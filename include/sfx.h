@@ -1,0 +1,127 @@
+/* sfx.h -- C ABI of the MI355X evaluator for sincFeyn's hot path: the hierarchical pruned Sinc sum over
+ * the z = log s lattice and the forest-subtracted integrand kernels at its points.
+ *
+ * Plain C types only (pointers, sizes, PODs): the C host layer (sincqed, src/sincqed.c), the Python
+ * bindings (ctypes) and any other caller bind to these symbols.  Every function returns 0 on success and
+ * a negative SFX_E* code on failure (never exit(), unlike the reference's CUDA back end, gpu.cu:16);
+ * sfx_last_error() describes the last failure of the calling thread.  Device memory is owned by the
+ * library; result structs and host arrays are owned by the caller.  Calls are made from one host
+ * thread at a time.
+ *
+ * Correspondence with the reference (reasther/sincFeyn v0.9.0):
+ *   sfx_init / sfx_info          gpu_setup's device choice and gpu_info       (driver/gpu.cu:30-34, gpu.h:21)
+ *   sfx_graph_*                  the graph registry graph_t / add_graph / find (driver/common.h:38,
+ *                                graphs.c:8-33) and its groups (graphs.c:15-29)
+ *   sfx_eval_nodes / _cells      gpu_submit + gpu_wait, and gpu_submit_cells  (driver/gpu.h:17-20,
+ *                                gpu_kernel.h:22-36: f w and Sum|terms| w per node, mode 0 / mode 1)
+ *   sfx_eval_raw                 the kernels' fd / fq at given s (common.h:27-29), golden primitive
+ *   sfx_hsum                     hsum() with lattice_setup()                  (driver/sum.c:556-814,
+ *                                lattice.c:13-27), symmetry fold off as in the reference GPU build (sum.c:571)
+ *   sfx_result_t                 res_t plus the counters the reference logs   (driver/sq.h:73, main.c:53-65)
+ */
+#ifndef SFX_H
+#define SFX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFX_OK          0
+#define SFX_EINVAL     -1   /* bad argument (unknown graph, size out of range, ...) */
+#define SFX_EDEVICE    -2   /* HIP runtime error */
+#define SFX_ENOMEM     -3   /* device or host allocation failed */
+#define SFX_ENOTINIT   -4   /* sfx_init not called */
+#define SFX_EUNSUP     -5   /* a mode this build does not support */
+
+typedef struct {
+    char name[32];        /* "g3_00" */
+    char volkov[64];      /* Volkov string, e.g. "2; 1-4, 3-6, 5-7" */
+    int L, D, weight;     /* loops, Sinc dimension, mirror weight (graph_t fields, common.h:38) */
+    int vgprs, sgprs, scratch_bytes, occupancy;   /* resources of the FP64 level kernel (informational) */
+} sfx_graph_info_t;
+
+/* Run options: the reference's environment knobs (main.c:176-185, sum.c:558-592), with its defaults
+ * filled by sfx_opts_default().  The all-DD error reference is abserr0 = 0, abrel = 0, abskip = 0
+ * (the reference's G2_ABSERR=0 SINCQED_ABREL=0 SINCQED_ABSKIP=0). */
+typedef struct {
+    double abserr0;       /* G2_ABSERR, default 1e-15 */
+    double abrel;         /* SINCQED_ABREL, default 1e-3 */
+    double nearf;         /* SINCQED_NEARF, default 1e-3 */
+    double ddslack;       /* SINCQED_DDSLACK, default 1 */
+    int abskip;           /* SINCQED_ABSKIP, default 1 */
+    int dropunres;        /* SINCQED_DROPUNRES, default 1 */
+    int keepbt;           /* SINCQED_KEEPBT: keep every grow boundary table (default 0) */
+    int btmargin;         /* SINCQED_BTMARGIN, default 2 */
+    long block_points;    /* children per refinement block on the device (0 = library default) */
+    int verbose;          /* 1: progress lines on stderr */
+} sfx_opts_t;
+
+typedef struct {
+    /* res_t (sq.h:73) */
+    double S[3];          /* S_h, S_2h, S_4h (NaN where the run has no such level) */
+    double Ah, ref, prune, trunc, unres, t;
+    long nev, nq, kept_top;
+    /* logged counters and the shadow sums (sum.c:74-81, 807-809; main.c:58-65) */
+    double es1, es2;      /* final-level sums of the points of shadow tier >= 1, >= 2 */
+    int esok;             /* shadow sums valid (top > 0) */
+    int zero;             /* box_zero: integrand identically zero at the probe nodes (sum.c:346-359) */
+    long abskipped;       /* evaluations that skipped the |.| sums */
+    long unresdrop;       /* points unresolved even in double-double, counted as 0 */
+    long btuns;           /* grow candidates that fell below the freed boundary tables (sumkey.h:452) */
+    long nev_level[16];   /* evaluations per level (index = level, top..0), grow at index top */
+    double abserr;        /* the fallback threshold used by the refinement */
+    int K, top, levels;   /* lattice half-width after rounding, coarsest level, levels run */
+    double t_grow, t_dilate, t_refine;   /* seconds */
+} sfx_result_t;
+
+/* ---- library ---- */
+int         sfx_init(int device);
+void        sfx_shutdown(void);
+const char *sfx_info(void);
+const char *sfx_last_error(void);
+void        sfx_opts_default(sfx_opts_t *o);
+
+/* ---- registry ---- */
+int sfx_graph_count(void);
+int sfx_graph_info(int index, sfx_graph_info_t *out);
+/* index of the graph with this name or Volkov string, or SFX_EINVAL (graphs.c:30-33) */
+int sfx_graph_find(const char *name_or_volkov);
+/* graphs.c:17-29 selection rule: 1 if graph `index` is selected by `sel` (name, Volkov string or group
+ * all2 all3 q3 vp3 lbl) */
+int sfx_graph_selected(int index, const char *sel);
+
+/* ---- per-point primitives (host buffers in, host buffers out; synchronous) ----
+ * Lattice: K = half-width, h = step; EX[k+K] = exp(k h) is built with the host's exp().
+ * sfx_eval_nodes: nodes k[i*D + d] (|k| <= K); fw[i] = f(s) w, abw[i] = Sum|terms| w with
+ *   w = prod(s_d h) in index order; skip[i] != 0 evaluates without the |.| sums (abw then undefined);
+ *   skip may be NULL.  This is the reference GPU kernel's mode 0 (gpu_kernel.h:32-35).
+ * sfx_eval_cells: the 2^D children k = cell - st b of nc cells, child b of cell c at (c << D) | b;
+ *   a child below the box gives 0, 0 (mode 1, gpu_kernel.h:27-31); skip is per cell. */
+int sfx_eval_nodes(int graph, double h, int K, long n, const int16_t *k, const uint8_t *skip,
+                   double *fw, double *abw);
+int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t *cells, const uint8_t *skip, int st,
+                   double *fw, double *abw);
+/* the integrand at given Sinc variables sf[i*D + d], no weight and no tests.  prec 0: f and Sum|terms|
+ * in FP64 into out[2i], out[2i+1]; prec 1: double-double, out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
+int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out);
+/* the per-point values with the reference's tests and double-double fallback (eval.c:33-61) as the
+ * refinement applies them: v[i] (0 if not finite), flags[i] = SFX_PT_* bits; abserr is the fallback
+ * threshold.  Returns the number of fallback points in *nq. */
+#define SFX_PT_NEAR 2
+#define SFX_PT_DD   4
+#define SFX_PT_UNRES 16
+int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, const uint8_t *skip,
+                      double abserr, const sfx_opts_t *opts, double *v, uint8_t *flags, long *nq);
+
+/* ---- the hierarchical sum ----
+ * hsum(g, h, N, eps, top) for one graph on the device: K = (N-1)/2 rounded up to a multiple of
+ * 2^(top+1) (lattice.c:16; K <= 127), top >= 1.  The coarsest level (support-following grow and the
+ * dilation) runs on the host with every point evaluated on the device; each refinement level runs on
+ * the device with the cell lists resident in HBM. */
+int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,51 @@
+/* oracle.h -- the CPU oracle: a restatement of the reference's hierarchical Sinc sum (TEST
+ * INFRASTRUCTURE ONLY -- tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it as the
+ * checker; the product library never links it).
+ *
+ * It restates, single-threaded and in plain C, reasther/sincFeyn v0.9.0:
+ *   driver/sum.c:556-814    hsum() with the symmetry fold, predictor, dump and histograms off
+ *   driver/sumkey.h:20-461  grow() and dilate() (64-bit grid keys)
+ *   driver/sum.c:346-359    box_zero()
+ *   driver/eval.c:25-95     finish(), qvalue(), leafx(), eval_batch() (fd per point; fd == fv bitwise)
+ *   driver/sum.c:18-20,76-81 Neumaier sums kadd/kval/kblock/kflush
+ *   driver/lattice.c:13-34  the EX table and lattice_node()
+ *   driver/main.c:46-48     value and error of a graph
+ * The integrand comes from the reference's own generated kernels, compiled where they lie into
+ * oracle/_ref/libsqk_ref.so (graph_t descriptors G_<name>: fd in double, fq in double-double).
+ */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double abserr0, abrel, nearf, ddslack;
+    int abskip, dropunres, keepbt, btmargin;
+    long maxpts;          /* nodes per refinement block: the reference CPU build's 2^18 (sum.c:643) */
+} orc_opts_t;
+
+typedef struct {
+    double S[3], Ah, ref, prune, trunc, unres;
+    long nev, nq, kept_top;
+    double es1, es2;
+    long abskipped, unresdrop, btuns;
+    long nev_level[16];
+    int zero, K, top;
+    double abserr;
+} orc_res_t;
+
+typedef struct { char volkov[64]; int L, D, weight, has_fv; } orc_graph_t;
+
+void orc_opts_default(orc_opts_t *o);
+int  orc_open(const char *kernels_so);                       /* 0 on success */
+int  orc_graph(const char *name, orc_graph_t *out);
+/* the reference kernel at Sinc variables sf[i*D + d]: fd (double half) -> f[i], ab[i] */
+int  orc_eval_fd(const char *name, long n, const double *sf, double *f, double *ab);
+/* fq (double-double half) -> out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
+int  orc_eval_fq(const char *name, long n, const double *sf, double *out);
+int  orc_hsum(const char *name, double h, int N, double eps, int top, const orc_opts_t *o, orc_res_t *r);
+const char *orc_error(void);
+
+#ifdef __cplusplus
+}
+#endif

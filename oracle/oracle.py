@@ -1,0 +1,139 @@
+"""oracle.py -- ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY: the checker used by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg; the product never imports it).
+
+  Oracle()                 liboracle.so (the C restatement of the reference's hsum, oracle/sq_oracle.c)
+                           on top of oracle/_ref/libsqk_ref.so (the reference's generated kernels)
+  run_reference(...)       oracle/_ref/sincqed_ref: the reference implementation itself, parsed
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, '_ref')
+LIB_ORACLE = os.path.join(HERE, '_build', 'liboracle.so')
+LIB_KERNELS = os.path.join(REF_DIR, 'libsqk_ref.so')
+LIB_KERNELS_NC = os.path.join(REF_DIR, 'libsqk_nc.so')
+SINCQED_REF = os.path.join(REF_DIR, 'sincqed_ref')
+
+
+class OrcOpts(C.Structure):
+    _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
+                ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
+                ('maxpts', C.c_long)]
+
+
+class OrcRes(C.Structure):
+    _fields_ = [('S', C.c_double * 3), ('Ah', C.c_double), ('ref', C.c_double), ('prune', C.c_double),
+                ('trunc', C.c_double), ('unres', C.c_double), ('nev', C.c_long), ('nq', C.c_long),
+                ('kept_top', C.c_long), ('es1', C.c_double), ('es2', C.c_double), ('abskipped', C.c_long),
+                ('unresdrop', C.c_long), ('btuns', C.c_long), ('nev_level', C.c_long * 16), ('zero', C.c_int),
+                ('K', C.c_int), ('top', C.c_int), ('abserr', C.c_double)]
+
+
+class OrcGraph(C.Structure):
+    _fields_ = [('volkov', C.c_char * 64), ('L', C.c_int), ('D', C.c_int), ('weight', C.c_int), ('has_fv', C.c_int)]
+
+
+def available():
+    return os.path.exists(LIB_ORACLE) and os.path.exists(LIB_KERNELS)
+
+
+class Oracle:
+    def __init__(self):
+        self.lib = C.CDLL(LIB_ORACLE)
+        L = self.lib
+        L.orc_open.argtypes = [C.c_char_p]
+        L.orc_graph.argtypes = [C.c_char_p, C.POINTER(OrcGraph)]
+        L.orc_eval_fd.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_eval_fq.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p]
+        L.orc_hsum.argtypes = [C.c_char_p, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(OrcOpts), C.POINTER(OrcRes)]
+        L.orc_opts_default.argtypes = [C.POINTER(OrcOpts)]
+        L.orc_error.restype = C.c_char_p
+        if L.orc_open(LIB_KERNELS.encode()) != 0:
+            raise RuntimeError(L.orc_error().decode())
+
+    def opts(self, **kw):
+        o = OrcOpts()
+        self.lib.orc_opts_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def graph(self, name):
+        g = OrcGraph()
+        if self.lib.orc_graph(name.encode(), C.byref(g)) != 0:
+            raise KeyError(name)
+        return dict(volkov=g.volkov.decode(), L=g.L, D=g.D, weight=g.weight, has_fv=g.has_fv)
+
+    def eval_fd(self, name, sf):
+        import numpy as np
+        sf = np.ascontiguousarray(sf, dtype=np.float64)
+        n = sf.shape[0]
+        f = np.zeros(n); ab = np.zeros(n)
+        if self.lib.orc_eval_fd(name.encode(), n, sf.ctypes.data, f.ctypes.data, ab.ctypes.data):
+            raise RuntimeError(self.lib.orc_error().decode())
+        return f, ab
+
+    def eval_fq(self, name, sf):
+        import numpy as np
+        sf = np.ascontiguousarray(sf, dtype=np.float64)
+        out = np.zeros((sf.shape[0], 4))
+        if self.lib.orc_eval_fq(name.encode(), sf.shape[0], sf.ctypes.data, out.ctypes.data):
+            raise RuntimeError(self.lib.orc_error().decode())
+        return out
+
+    def hsum(self, name, h, N, eps, top, **kw):
+        r = OrcRes()
+        o = self.opts(**kw)
+        if self.lib.orc_hsum(name.encode(), h, N, eps, top, C.byref(o), C.byref(r)) != 0:
+            raise RuntimeError(self.lib.orc_error().decode())
+        return res_dict(r)
+
+
+def res_dict(r):
+    d = {k: getattr(r, k) for k, _ in r._fields_ if k not in ('S', 'nev_level')}
+    d['S'] = list(r.S)
+    d['nev_level'] = list(r.nev_level)
+    return d
+
+
+def default_top(h, topstep=3.0):
+    """main.c:211-215"""
+    top = 1
+    while top < 2 and (1 << (top + 1))*h <= topstep*(1 + 1e-9):
+        top += 1
+    return top
+
+
+_RLINE = re.compile(r'^#R (\S+) (.*)$')
+
+
+def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=600):
+    """the reference implementation (oracle/_ref/sincqed_ref) -> {graph: {field: value}}, wall seconds"""
+    e = dict(os.environ)
+    e.update({'SINCQED_CKPT': '0', 'SINCQED_PROGRESS': '0', 'SINCQED_RESULTS': '-'})
+    if threads:
+        e['OMP_NUM_THREADS'] = str(threads)
+    if env:
+        e.update(env)
+    args = [SINCQED_REF, 'run', repr(float(h)), str(N), repr(float(eps))]
+    if top is not None:
+        args.append(str(top))
+    args += list(graphs)
+    p = subprocess.run(args, env=e, capture_output=True, text=True, timeout=timeout)
+    if p.returncode != 0:
+        raise RuntimeError('sincqed_ref failed: %s' % p.stderr[-2000:])
+    out, wall = {}, None
+    for line in p.stdout.splitlines():
+        m = _RLINE.match(line)
+        if m:
+            toks = m.group(2).split()
+            d = {}
+            for k, v in zip(toks[0::2], toks[1::2]):
+                d[k] = float(v) if any(c in v for c in '.en') else int(v)
+            out[m.group(1)] = d
+        elif line.startswith('# wall'):
+            wall = float(line.split()[2])
+    return out, wall, p.stdout

@@ -1,0 +1,73 @@
+// sfx_device.hpp -- the library's device state and the graph-independent device operations, shared by
+// the runtime translation units (sfx_runtime.hip, sfx_hsum.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "sfx_integrand.hpp"
+#include "../../include/sfx.h"
+
+extern "C" const sfx_graph_entry *const SFX_GRAPHS[];
+extern "C" const int SFX_NGRAPHS;
+
+namespace sfx {
+
+// error reporting: per calling thread, returned by sfx_last_error()
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define SFX_HIP(x) do{ hipError_t e_ = (x); if(e_ != hipSuccess) return sfx::hip_fail(e_, #x, __FILE__, __LINE__); }while(0)
+
+struct Lib {
+    bool init = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop;
+    std::string info;
+};
+Lib &lib();
+int require_init();
+
+// a device buffer that grows on demand (never shrinks while the library is up)
+struct DevBuf {
+    void *p = nullptr; size_t cap = 0;
+    int reserve(size_t bytes);
+    void release();
+    template<class T> T *as() const { return (T*)p; }
+};
+
+// double-double accumulator as two doubles (device and host)
+struct ddacc { double hi, lo; };
+
+// per-tile partial of a refinement block (see k_level_partials)
+struct LevelPart { double s[2], a[2], b1[2], b2[2], drop[2]; uint32_t nv, nskip, nkeep, pad; };
+// level accumulator in device memory, added to by every block of the level in block order
+struct LevelAcc {
+    double s[2], a[2], b1[2], b2[2], drop[2], u[2];
+    unsigned long long nv, nskip, nkept, nq, nd;
+};
+
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_ITEMS = 8;                       // items per thread
+constexpr long TILE = (long)TILE_THREADS*TILE_ITEMS;
+
+// ordered compaction: out[0..*count-1] = the indices i < n with (fl[i] & mask) != 0, ascending.
+// Scratch: blk (2 * ceil(n/TILE) + 1 uint32).
+hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask, uint32_t *blk, uint32_t *out, uint32_t *count);
+
+// the sums and the kept children of one refinement block (reference sum.c:726-746): for every valid
+// child, in child order, S += v pw, A += |v|, skipped += skip; at the last level the shadow-tier sums;
+// at other levels the children with |v| >= thr become cells of the next level (flags from the parent's
+// skip bit, the child's near flag and tier, sum.c:744) and the others add |v| pw to the dropped weight.
+// The double-double fallback results (e2, nd of the ddcnt fallback points) are reduced too.  Everything
+// is added to *acc in a fixed order; nxt receives the kept cells at offset acc->nkept (before the call).
+struct LevelArgs {
+    const double *v; const uint8_t *fl; const sfx_cell_t *cells; long n; int D; int st; int last; int abskip;
+    double pw, thr;
+    const double *e2; const uint8_t *nd; const uint32_t *ddcnt;
+    sfx_cell_t *nxt; long nxt_cap;
+    LevelPart *part; uint32_t *keepoff;             // scratch: ceil(n/TILE) entries each
+    LevelAcc *acc;
+};
+hipError_t level_reduce(hipStream_t s, const LevelArgs &a);
+
+}  // namespace sfx

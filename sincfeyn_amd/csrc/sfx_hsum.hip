@@ -1,0 +1,427 @@
+// sfx_hsum.hip -- the hierarchical Sinc sum of one graph (reference driver/sum.c:556-814 with the
+// symmetry fold off, as the reference's GPU build runs it, sum.c:571).
+//
+//  * coarsest level (step 2^top h): the support-following grow (reference driver/sumkey.h:130-461) on the
+//    host, every frontier evaluated on the device (k_eval nodes + ordered fallback list + k_dd); then the
+//    dilation (sumkey.h:20-90) on the host; the cells go to HBM once;
+//  * refinement levels top-1 .. 0: entirely on the device, in blocks of cells -- children made in the
+//    kernel, FP64 evaluation with the per-point tests, the double-double recompute of the flagged points,
+//    deterministic double-double block sums, and the ordered compaction of the kept children into the
+//    next level's cell list, which never leaves HBM.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <chrono>
+#include <vector>
+#include "sfx_device.hpp"
+
+namespace sfx {
+
+static double now(){ return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+struct JobBufs {
+    DevBuf EX, pts, v, fl, ddblk, ddidx, ddcnt, e2, nd, cur, nxt, part, keepoff, acc;
+    void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc};
+                    for(DevBuf *b : all) b->release(); }
+};
+static JobBufs &bufs(){ static JobBufs B; return B; }
+
+// Neumaier compensated sum (reference sum.c:18-20)
+struct Neu {
+    double s = 0, c = 0;
+    void add(double x){ double t = s + x; if(fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s; s = t; }
+    double val() const { return s + c; }
+};
+
+// open-addressing map from a nonzero 64-bit key to a 32-bit value (linear probing, power-of-two size)
+struct KeyMap {
+    std::vector<uint64_t> key; std::vector<int32_t> val; size_t mask = 0, n = 0;
+    explicit KeyMap(size_t want = 1024){ reset(want); }
+    void reset(size_t want){
+        size_t c = 1024; while(c < 2*want) c <<= 1;
+        key.assign(c, 0); val.assign(c, 0); mask = c - 1; n = 0;
+    }
+    static size_t h(uint64_t y){ return (size_t)((y*0x9E3779B97F4A7C15ULL) >> 21); }
+    void grow(){
+        std::vector<uint64_t> ok; std::vector<int32_t> ov; ok.swap(key); ov.swap(val);
+        size_t c = (mask + 1)*2; key.assign(c, 0); val.assign(c, 0); mask = c - 1; n = 0;
+        for(size_t i = 0; i < ok.size(); i++) if(ok[i]) *insert(ok[i], nullptr) = ov[i];
+    }
+    // slot of y (inserted with value 0 if absent; *fresh = 1 then)
+    int32_t *insert(uint64_t y, int *fresh){
+        if(2*(n + 1) > mask + 1) grow();
+        size_t i = h(y) & mask;
+        while(key[i] && key[i] != y) i = (i + 1) & mask;
+        if(!key[i]){ key[i] = y; val[i] = 0; n++; if(fresh) *fresh = 1; } else if(fresh) *fresh = 0;
+        return &val[i];
+    }
+    const int32_t *find(uint64_t y) const {
+        size_t i = h(y) & mask;
+        while(key[i]){ if(key[i] == y) return &val[i]; i = (i + 1) & mask; }
+        return nullptr;
+    }
+};
+
+struct Job {
+    const sfx_graph_entry *g; int gi, D, K, top; double h, eps;
+    sfx_opts_t o; sfx_eval_params_t P;
+    hipStream_t s;
+};
+
+// ---------------------------------------------------------------------------------------------------
+// evaluate a list of nodes on the device with the per-point tests and the fallback (leafx semantics,
+// eval.c:33-60): v, flags (SFX_PT_*), plus the fallback count, the round column's e^2 and the
+// unresolved count, accumulated in list order
+static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v, uint8_t *flg, long *nq, double *u, long *nd){
+    if(n <= 0) return SFX_OK;
+    JobBufs &B = bufs(); int rc;
+    long nb = (n + TILE - 1)/TILE;
+    if((rc = B.pts.reserve(n*sizeof(sfx_cell_t))) || (rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) ||
+       (rc = B.ddblk.reserve((2*nb + 2)*sizeof(uint32_t))) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
+       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double))) || (rc = B.nd.reserve(n))) return rc;
+    hipStream_t s = J.s;
+    SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
+    SFX_HIP(J.g->eval(s, 0, B.pts.as<sfx_cell_t>(), n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+    SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
+    SFX_HIP(J.g->dd(s, 0, B.pts.as<sfx_cell_t>(), 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
+                    B.v.as<double>(), B.e2.as<double>(), B.nd.as<uint8_t>()));
+    uint32_t m = 0;
+    std::vector<uint8_t> fl(n);
+    SFX_HIP(hipMemcpyAsync(v, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(fl.data(), B.fl.p, n, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(&m, B.ddcnt.p, sizeof m, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    std::vector<double> e2(m); std::vector<uint8_t> ndv(m); std::vector<uint32_t> idx(m);
+    if(m){
+        SFX_HIP(hipMemcpyAsync(e2.data(), B.e2.p, m*sizeof(double), hipMemcpyDeviceToHost, s));
+        SFX_HIP(hipMemcpyAsync(ndv.data(), B.nd.p, m, hipMemcpyDeviceToHost, s));
+        SFX_HIP(hipMemcpyAsync(idx.data(), B.ddidx.p, m*sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SFX_HIP(hipStreamSynchronize(s));
+    }
+    for(long i = 0; i < n; i++) flg[i] = fl[i] & (SFX_FL_NEAR | SFX_FL_DD);
+    for(uint32_t j = 0; j < m; j++){ *u += e2[j]; if(ndv[j]){ (*nd)++; flg[idx[j]] |= SFX_PT_UNRES; } }
+    *nq += m;
+    return SFX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// grid keys: the mixed-radix number 1 c_0 .. c_{D-1} in base W = 2 (K/st) + 1, c_i = k_i/st + K/st, so that
+// numeric order is lexicographic order in k (the reference's key, sumkey.h:7-18)
+struct Grid {
+    int D, K, st, W; uint64_t P[SFX_MAXD];
+    bool init(int D_, int K_, int st_){
+        D = D_; K = K_; st = st_; W = 2*(K/st) + 1;
+        unsigned __int128 p = 1;
+        for(int i = D - 1; i >= 0; i--){ P[i] = (uint64_t)p; p *= W; }
+        return p < ((unsigned __int128)1 << 62);
+    }
+    uint64_t key(const int8_t *k) const { uint64_t y = 1; for(int i = 0; i < D; i++) y = y*W + (uint64_t)(k[i]/st + K/st); return y; }
+    void decode(uint64_t y, int8_t *k) const { for(int i = D - 1; i >= 0; i--){ k[i] = (int8_t)(((int)(y % W) - K/st)*st); y /= W; } }
+};
+
+struct GrowOut {
+    std::vector<sfx_cell_t> keep;
+    double S = 0, A = 0, e1 = 0, e2 = 0, drop = 0, u = 0; long nev = 0, nq = 0, nd = 0, btuns = 0;
+};
+
+// the support-following grow at step st = 2^lev (lev >= 1), reference sumkey.h:130-461: frontier L+1 is the
+// axis neighbours of frontier L's kept points not evaluated before, in the order of a pass over (kept
+// point, axis, sign); a neighbour is known as evaluated when it is a kept point of frontier L-1 or a
+// boundary point of a frontier L' (n <= L' <= L-1, same parity as its norm n) whose table has not been
+// freed.  Every frontier is summed in order (Neumaier for S, plain for the reference scale A); points
+// with |v| >= eps_rel A (running A) are kept.  At the end the final threshold is re-applied and the
+// shadow tiers are flood-filled.
+static int grow(Job &J, int lev, double eps_rel, GrowOut &G){
+    const int D = J.D, K = J.K, st = 1 << lev;
+    Grid gr; if(!gr.init(D, K, st)){ set_error("grow: W^D does not fit 64-bit keys (D=%d, W=%d)", D, 2*(K/st) + 1); return SFX_EUNSUP; }
+    const double pw = pow((double)st, (double)D);
+    std::vector<sfx_cell_t> fr(1), nx; memset(&fr[0], 0, sizeof(sfx_cell_t));
+    std::vector<double> kv; std::vector<uint64_t> pk;           // |v| of kept points; keys of the previous frontier's kept points
+    KeyMap bmap(1 << 16);                                          // boundary point -> its frontier
+    Neu ks; double a = 0, x1 = 0, x2 = 0, dr = 0;
+    int lay = 0; long BTLO = 0;
+    std::vector<double> v; std::vector<uint8_t> flg;
+    while(!fr.empty()){
+        long n = (long)fr.size();
+        v.resize(n); flg.resize(n);
+        int rc = eval_nodes_finished(J, fr.data(), n, v.data(), flg.data(), &G.nq, &G.u, &G.nd);
+        if(rc) return rc;
+        G.nev += n;
+        for(long i = 0; i < n; i++){ ks.add(v[i]*pw); a += fabs(v[i])*pw; }
+        const double thr = eps_rel*a;
+        std::vector<long> kj;
+        for(long i = 0; i < n; i++){
+            int mx = 0; for(int d = 0; d < D; d++){ int x = abs((int)fr[i].k[d]); if(x > mx) mx = x; }
+            double w = fabs(v[i])*pw; bool kp = fabs(v[i]) >= thr;
+            if(mx == K) x1 += w; else if(mx == K - st) x2 += w;
+            if(!kp){ dr += w; *bmap.insert(gr.key(fr[i].k), nullptr) = lay; }
+            else {
+                kj.push_back(i);
+                sfx_cell_t c = fr[i]; c.fl = (J.o.abskip ? (uint8_t)!(flg[i] & SFX_PT_NEAR) : 0);
+                G.keep.push_back(c); kv.push_back(fabs(v[i]));
+            }
+        }
+        // candidates of the next frontier
+        KeyMap hs(pk.size() + kj.size()*2*D);
+        for(uint64_t y : pk) *hs.insert(y, nullptr) = 1;
+        nx.clear();
+        for(long j : kj){
+            const sfx_cell_t &p = fr[j]; uint64_t y0 = gr.key(p.k);
+            int nrm = 0; for(int d = 0; d < D; d++) nrm += abs((int)p.k[d]);
+            nrm /= st;
+            for(int d = 0; d < D; d++) for(int sg = -1; sg <= 1; sg += 2){
+                int kk = p.k[d] + st*sg;
+                if(kk > K || kk < -K) continue;
+                uint64_t y = sg > 0 ? y0 + gr.P[d] : y0 - gr.P[d];
+                int n1 = abs(kk) > abs((int)p.k[d]) ? nrm + 1 : nrm - 1;
+                if(n1 < BTLO) G.btuns++;
+                const int32_t *bl = bmap.find(y);
+                if(bl && *bl >= n1 && *bl < lay && ((*bl - n1) & 1) == 0 && *bl >= BTLO) continue;
+                int fresh; hs.insert(y, &fresh);
+                if(!fresh) continue;
+                sfx_cell_t c = p; c.k[d] = (int8_t)kk; c.fl = 0;
+                nx.push_back(c);
+            }
+        }
+        pk.clear(); for(long j : kj) pk.push_back(gr.key(fr[j].k));
+        if(!nx.empty() && !J.o.keepbt){
+            long mn = 1L << 40;
+            for(const sfx_cell_t &c : nx){ long t = 0; for(int d = 0; d < D; d++) t += abs((int)c.k[d]); t /= st; if(t < mn) mn = t; }
+            mn -= J.o.btmargin;
+            if(mn - 1 > BTLO) BTLO = mn - 1 < lay + 1 ? mn - 1 : lay + 1;
+        }
+        fr.swap(nx); lay++;
+    }
+    // the final threshold (sumkey.h:397-401)
+    const double thr = eps_rel*a;
+    long m = 0;
+    for(size_t i = 0; i < G.keep.size(); i++){
+        if(kv[i] >= thr){ kv[m] = kv[i]; G.keep[m] = G.keep[i]; m++; } else dr += kv[i]*pw;
+    }
+    G.keep.resize(m); kv.resize(m);
+    // shadow tiers t = 1, 2: reached from z = 0 through kept points with |f| >= 10^t thr (sumkey.h:402-453)
+    if(m && D < 12){
+        KeyMap idx(m);
+        for(long i = 0; i < m; i++) *idx.insert(gr.key(G.keep[i].k), nullptr) = (int32_t)i;
+        std::vector<uint8_t> tr(m, 0); std::vector<long> cu, nw;
+        for(int t = 1; t <= 2; t++){
+            double lim = (t == 1 ? 10 : 100)*thr;
+            cu.clear();
+            for(long i = 0; i < m; i++){
+                bool z = true; for(int d = 0; d < D; d++) if(G.keep[i].k[d]){ z = false; break; }
+                if(z && kv[i] >= lim){ tr[i] = (uint8_t)t; cu.push_back(i); }
+            }
+            while(!cu.empty()){
+                nw.clear();
+                for(long i : cu){
+                    uint64_t y0 = gr.key(G.keep[i].k);
+                    for(int d = 0; d < D; d++) for(int sg = -1; sg <= 1; sg += 2){
+                        int kk = G.keep[i].k[d] + st*sg;
+                        if(kk > K || kk < -K) continue;
+                        const int32_t *q = idx.find(sg > 0 ? y0 + gr.P[d] : y0 - gr.P[d]);
+                        if(q && kv[*q] >= lim && tr[*q] < t){ tr[*q] = (uint8_t)t; nw.push_back(*q); }
+                    }
+                }
+                cu.swap(nw);
+            }
+        }
+        for(long i = 0; i < m; i++) G.keep[i].fl = (uint8_t)((G.keep[i].fl & 1) | (tr[i] << 1));
+    }
+    G.S = ks.val(); G.A = a; G.e1 = x1; G.e2 = x2; G.drop = dr;
+    return SFX_OK;
+}
+
+// the axis neighbours of every kept cell at step st, merged (skip bit: AND, tier: max) and sorted in
+// lexicographic order of k (reference sumkey.h:20-90)
+static int dilate(const Job &J, std::vector<sfx_cell_t> &cells, int st){
+    Grid gr; if(!gr.init(J.D, J.K, st)){ set_error("dilate: keys do not fit"); return SFX_EUNSUP; }
+    const int D = J.D, K = J.K;
+    KeyMap m(cells.size()*(2*D + 1));
+    for(const sfx_cell_t &c : cells){
+        int sk = c.fl & 1, tr = c.fl >> 1; uint64_t y0 = gr.key(c.k);
+        for(int j = -1; j < D; j++) for(int sg = -1; sg <= 1; sg += 2){
+            if(j < 0 && sg > 0) continue;
+            uint64_t y = y0;
+            if(j >= 0){ int kk = c.k[j] + st*sg; if(kk > K || kk < -K) continue; y = sg > 0 ? y + gr.P[j] : y - gr.P[j]; }
+            int fresh; int32_t *e = m.insert(y, &fresh);
+            int esk = fresh ? 1 : (*e & 1), etr = fresh ? 0 : (*e >> 1);
+            if(!sk) esk = 0;
+            if(tr > etr) etr = tr;
+            *e = esk | (etr << 1);
+        }
+    }
+    std::vector<uint64_t> keys; keys.reserve(m.n);
+    for(size_t i = 0; i <= m.mask; i++) if(m.key[i]) keys.push_back(m.key[i]);
+    std::sort(keys.begin(), keys.end());
+    cells.resize(keys.size());
+    for(size_t j = 0; j < keys.size(); j++){
+        sfx_cell_t &c = cells[j]; memset(&c, 0, sizeof c);
+        gr.decode(keys[j], c.k);
+        c.fl = (uint8_t)*m.find(keys[j]);
+    }
+    return SFX_OK;
+}
+
+// box_zero (reference sum.c:346-359): z = 0 and 4096 pseudo-random nodes of the step-2^lev lattice; the
+// node sequence is the reference's (a multiplicative hash of the probe number)
+static int box_zero(Job &J, int lev, int *zero, long *nev, long *unresdrop){
+    const int st = 1 << lev, W = 2*(J.K/st) + 1, NP = 4096;
+    std::vector<sfx_cell_t> p(NP + 1);
+    for(int i = 0; i <= NP; i++){
+        memset(&p[i], 0, sizeof p[i]);
+        unsigned long long y = 0x9E3779B97F4A7C15ULL*(unsigned long long)(i + 1);
+        if(i) for(int d = 0; d < J.D; d++){ y ^= y >> 29; y *= 0xBF58476D1CE4E5B9ULL; y ^= y >> 32; p[i].k[d] = (int8_t)(((int)(y % (unsigned long long)W) - J.K/st)*st); }
+    }
+    std::vector<double> v(NP + 1); std::vector<uint8_t> f(NP + 1); long q = 0, nd = 0; double u = 0;
+    int rc = eval_nodes_finished(J, p.data(), NP + 1, v.data(), f.data(), &q, &u, &nd);
+    if(rc) return rc;
+    *zero = 1; *unresdrop += nd;          // UNRESDROP is global in the reference; the probe's q and u are local
+    for(int i = 0; i <= NP; i++) if(v[i] != 0){ *zero = 0; break; }
+    if(*zero) *nev += NP + 1;
+    return SFX_OK;
+}
+
+static void set_params(Job &J, double abserr){
+    J.P.K = J.K; J.P.H = J.h; J.P.abskip = J.o.abskip; J.P.abserr = abserr; J.P.nearf = J.o.nearf;
+    J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.EX = bufs().EX.as<double>();
+}
+
+__global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0) memset(a, 0, sizeof *a); }
+
+// one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
+static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
+    JobBufs &B = bufs(); hipStream_t s = J.s; int rc;
+    const int D = J.D, st = 1 << lev; const bool last = lev == 0;
+    const double pw = pow((double)st, (double)D), thr = J.eps*ref*1e-2;
+    long maxpts = J.o.block_points > 0 ? J.o.block_points : (1L << 24);
+    long Bc = maxpts >> D; if(Bc < 1) Bc = 1;
+    long nmax = Bc << D, nbt = (nmax + TILE - 1)/TILE;
+    if((rc = B.v.reserve(nmax*sizeof(double))) || (rc = B.fl.reserve(nmax)) || (rc = B.ddblk.reserve((2*nbt + 2)*sizeof(uint32_t))) ||
+       (rc = B.ddidx.reserve(nmax*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(nmax*sizeof(double))) ||
+       (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
+       (rc = B.acc.reserve(sizeof(LevelAcc)))) return rc;
+    hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
+    long kept = 0;
+    for(long c0 = 0; c0 < ncur; c0 += Bc){
+        long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
+        const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
+        SFX_HIP(J.g->eval(s, 1, cells, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+        SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
+        SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P, B.v.as<double>(),
+                        B.e2.as<double>(), B.nd.as<uint8_t>()));
+        if(!last){
+            // room for every child of this block in the next level's list (kept so far is known exactly)
+            if((rc = B.nxt.reserve((size_t)(kept + n)*sizeof(sfx_cell_t)))){
+                // a reallocation loses the kept cells: grow by copying through a temporary buffer
+                return rc;
+            }
+        }
+        LevelArgs a;
+        a.v = B.v.as<double>(); a.fl = B.fl.as<uint8_t>(); a.cells = cells; a.n = n; a.D = D; a.st = st; a.last = last;
+        a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nd = B.nd.as<uint8_t>();
+        a.ddcnt = B.ddcnt.as<uint32_t>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
+        a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<uint32_t>(); a.acc = B.acc.as<LevelAcc>();
+        SFX_HIP(level_reduce(s, a));
+        if(!last){
+            unsigned long long k = 0;
+            SFX_HIP(hipMemcpyAsync(&k, (char*)B.acc.p + offsetof(LevelAcc, nkept), sizeof k, hipMemcpyDeviceToHost, s));
+            SFX_HIP(hipStreamSynchronize(s));
+            kept = (long)k;
+        }
+        if(J.o.verbose) fprintf(stderr, "  level %d: cells %ld/%ld\n", lev, c0 + nc, ncur);
+    }
+    SFX_HIP(hipMemcpyAsync(out, B.acc.p, sizeof *out, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    *nnext = (long)out->nkept;
+    return SFX_OK;
+}
+
+// keep the device list `nxt` when it must grow: DevBuf::reserve does not preserve contents, so the
+// next-level list is sized before the level starts to the bound ncur << D when that fits, else grown
+// with a copy (see refine_level_checked)
+static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s){
+    if(want <= b.cap) return SFX_OK;
+    DevBuf t; int rc = t.reserve(want); if(rc) return rc;
+    if(used) SFX_HIP(hipMemcpyAsync(t.p, b.p, used, hipMemcpyDeviceToDevice, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    b.release(); b = t;
+    return SFX_OK;
+}
+
+}  // namespace sfx
+
+using namespace sfx;
+
+extern "C" void sfx_release_job_buffers(void){ bufs().release(); }
+
+extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
+    int rc;
+    if((rc = require_init())) return rc;
+    if(graph < 0 || graph >= SFX_NGRAPHS || !res){ set_error("sfx_hsum: bad graph or result pointer"); return SFX_EINVAL; }
+    if(top < 1 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("sfx_hsum: need h > 0, N >= 3, eps > 0, 1 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
+    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.eps = eps; J.top = top; J.s = lib().stream;
+    if(opts) J.o = *opts; else sfx_opts_default(&J.o);
+    int m = 1 << (top + 1); J.K = (N - 1)/2; J.K = ((J.K + m - 1)/m)*m;      // lattice.c:16
+    if(J.K > 127){ set_error("sfx_hsum: N = %d gives K = %d; points are stored in 8 bits, so K <= 127 (N <= 255)", N, J.K); return SFX_EINVAL; }
+    memset(res, 0, sizeof *res);
+    res->K = J.K; res->top = top; res->S[1] = res->S[2] = NAN;
+    JobBufs &B = bufs();
+    double t0 = now();
+    // the EX table on the host with exp() (lattice.c:18-19), uploaded as is
+    std::vector<double> EX(2*J.K + 1);
+    for(int k = -J.K; k <= J.K; k++) EX[k + J.K] = exp(k*h);
+    if((rc = B.EX.reserve(EX.size()*sizeof(double)))) return rc;
+    SFX_HIP(hipMemcpyAsync(B.EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
+    set_params(J, J.o.abserr0);
+    int zero = 0;
+    if((rc = box_zero(J, top, &zero, &res->nev, &res->unresdrop))) return rc;
+    if(zero){ res->zero = 1; res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
+    GrowOut G;
+    if((rc = grow(J, top, eps*1e-3, G))) return rc;
+    res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
+    res->nev_level[top] = G.nev;
+    res->ref = G.A; res->prune = G.drop;
+    double t1 = now(); res->t_grow = t1 - t0;
+    if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
+    { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
+    double Slev[16]; int nl = 0; Slev[nl++] = G.S;
+    std::vector<sfx_cell_t> &cells = G.keep;
+    if((rc = dilate(J, cells, 1 << top))) return rc;
+    res->kept_top = (long)cells.size();
+    double t2 = now(); res->t_dilate = t2 - t1;
+    // refinement on the device
+    long ncur = (long)cells.size();
+    if((rc = B.cur.reserve((size_t)(ncur ? ncur : 1)*sizeof(sfx_cell_t)))) return rc;
+    if(ncur) SFX_HIP(hipMemcpyAsync(B.cur.p, cells.data(), ncur*sizeof(sfx_cell_t), hipMemcpyHostToDevice, J.s));
+    double abserr = fmax(J.o.abserr0, J.o.abrel*eps*G.A);          // sum.c:651
+    res->abserr = abserr;
+    set_params(J, abserr);
+    res->esok = J.D < 12;
+    for(int lev = top - 1; lev >= 0; lev--){
+        long nnext = 0; LevelAcc A;
+        if(lev > 0){
+            // the next level's list: at most every child of every cell
+            size_t bound = (size_t)ncur << J.D;
+            if((rc = reserve_preserving(B.nxt, 0, bound*sizeof(sfx_cell_t), J.s))) return rc;
+        }
+        if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))) return rc;
+        double S = A.s[0] + A.s[1];
+        res->prune += A.drop[0] + A.drop[1];
+        res->nev += (long)A.nv; res->nq += (long)A.nq; res->unresdrop += (long)A.nd;
+        res->abskipped += (long)A.nskip;
+        res->nev_level[lev] = (long)A.nv;
+        Slev[nl++] = S;
+        if(lev == 0){
+            res->unres = sqrt(A.u[0] + A.u[1]); res->Ah = A.a[0] + A.a[1];
+            res->es1 = A.b1[0] + A.b1[1]; res->es2 = A.b2[0] + A.b2[1];
+        }
+        if(lev > 0){ std::swap(B.cur, B.nxt); ncur = nnext; }
+    }
+    res->levels = nl;
+    res->S[0] = Slev[nl - 1]; res->S[1] = nl > 1 ? Slev[nl - 2] : NAN; res->S[2] = nl > 2 ? Slev[nl - 3] : NAN;
+    res->t_refine = now() - t2;
+    res->t = now() - t0;
+    return SFX_OK;
+}

@@ -1,0 +1,192 @@
+// sfx_integrand.hpp -- included by every emitted integrand translation unit (k_<graph>.hip).
+//
+// It instantiates, for one graph, the device kernels that evaluate the integrand at lattice points:
+//   k_eval<G, CELLS>  one thread per point: s = e^{k h} from the job's table, the weight w = prod(s_i h)
+//                     in index order (reference driver/lattice.c:30-34), f in FP64, and the reference's
+//                     per-point tests (driver/eval.c:33-53): the value f w (0 if not finite), the "near"
+//                     flag and the fallback flag.  CELLS = 1: point i is child b = i mod 2^D of cell
+//                     c = i / 2^D at k = cell - st b (reference sum.c:405-408); a child below the box is
+//                     not a lattice node (flag VALID clear, value 0).
+//   k_dd<G, CELLS>    one thread per flagged point (an index list): the same point in double-double,
+//                     then the reference's qvalue (eval.c:25-30): value, eps^2 term of the round column,
+//                     and the "unresolved" mark.
+//   k_raw<G>          golden primitive: f and Sum|terms| at nodes, in FP64 (and in DD), without the tests.
+// The FP64 path is chosen per wave: when every lane's point skips the |.| sums the integrand is
+// instantiated with SQ_AB = 0 (those sums are then dead code), else with SQ_AB = 1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+#include "sfx_dd.hpp"
+#include "sfx_abi_types.h"
+
+#define SFX_FN __device__ __forceinline__
+#define SFX_CHUNK __device__ __noinline__
+
+#ifndef SFX_BLOCK
+#define SFX_BLOCK 256
+#endif
+#define SFX_QEPS 4.93e-32          /* relative precision of the double-double type (reference common.h:7) */
+
+namespace sfx {
+
+// coordinates of point i: node mode reads record i; cell mode derives child b of cell c
+template<int D, bool CELLS>
+__device__ __forceinline__ bool point_coords(const sfx_cell_t *__restrict__ pts, long i, int st, int K, int *k, unsigned *cfl){
+    if(CELLS){
+        long c = i >> D; int b = (int)(i & ((1 << D) - 1));
+        const sfx_cell_t cell = pts[c];
+        *cfl = cell.fl;
+        bool ok = true;
+#pragma unroll
+        for(int d = 0; d < D; d++){ int kd = (int)cell.k[d] - (((b >> d) & 1) ? st : 0); k[d] = kd; ok = ok && kd >= -K; }
+        return ok;
+    } else {
+        const sfx_cell_t p = pts[i];
+        *cfl = p.fl;
+#pragma unroll
+        for(int d = 0; d < D; d++) k[d] = p.k[d];
+        return true;
+    }
+}
+
+template<int D>
+__device__ __forceinline__ double lattice_node(const double *__restrict__ EX, int K, double H, const int *k, double *sf){
+    double w = 1;
+#pragma unroll
+    for(int d = 0; d < D; d++){ double s = EX[k[d] + K]; sf[d] = s; w *= s*H; }
+    return w;
+}
+
+template<class G, bool CELLS>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_eval(const sfx_cell_t *__restrict__ pts, long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout){
+    constexpr int D = G::D;
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    bool in = i < n;
+    int k[D]; unsigned cfl = 0; bool valid = false;
+    if(in) valid = point_coords<D, CELLS>(pts, i, st, P.K, k, &cfl);
+    bool skip = valid && P.abskip && (cfl & 1u);
+    // the |.| sums are skipped only when every lane of the wave may skip them
+    bool allskip = __all(!valid || skip);
+    double v = 0; unsigned fl = 0;
+    if(valid){
+        double sf[D]; double w = lattice_node<D>(P.EX, P.K, P.H, k, sf);
+        double ab = 0, f;
+        if(allskip) f = G::template f<double>(sf, &ab, 0);
+        else f = G::template f<double>(sf, &ab, 1);
+        v = f*w; fl = SFX_FL_VALID;
+        if(skip){ fl |= SFX_FL_SKIP; if(!isfinite(v)) v = 0; }
+        else {
+            double bnd = ab*w*16*DBL_EPSILON/P.ddslack;            // eval.c:36, same association
+            if(!(bnd <= P.nearf*P.abserr)) fl |= SFX_FL_NEAR;
+            if(!(bnd <= P.abserr)){ fl |= SFX_FL_DD; v = 0; }
+            else if(!isfinite(v)) v = 0;
+        }
+    }
+    if(in){ vout[i] = v; flout[i] = (uint8_t)fl; }
+}
+
+// the double-double recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
+// the count); writes the value, the eps^2 contribution to the round column and the "unresolved" mark
+template<class G, bool CELLS>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ cnt,
+     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ ndout){
+    constexpr int D = G::D;
+    long m = *cnt;
+    for(long j = blockIdx.x*(long)blockDim.x + threadIdx.x; j < m; j += (long)gridDim.x*blockDim.x){
+        long i = idx[j];
+        int k[D]; unsigned cfl;
+        point_coords<D, CELLS>(pts, i, st, P.K, k, &cfl);
+        double sf[D]; double w = lattice_node<D>(P.EX, P.K, P.H, k, sf);
+        sfx_dd sq[D];
+#pragma unroll
+        for(int d = 0; d < D; d++) sq[d] = sfx_dd(sf[d]);
+        sfx_dd abq;
+        sfx_dd vq = G::template f<sfx_dd>(sq, &abq, 1);
+        double vqd = (double)vq, abqd = (double)abq;
+        double e = abqd*w*SFX_QEPS;                                   // eval.c:26
+        double e2 = (e*16 <= P.abserr) ? 0.0 : (isfinite(e) ? e*e : 0.0);
+        double v = isfinite(vqd) ? vqd*w : 0;
+        uint8_t nd = 0;
+        if(!(e <= fabs(v))){ nd = 1; if(P.dropunres) v = 0; }
+        vout[i] = v; e2out[j] = e2; ndout[j] = nd;
+    }
+}
+
+// golden primitive: the integrand at nodes without the tests.  prec 0: o1 = f, o2 = Sum|terms| in FP64
+// (SQ_AB = 1); prec 1: o1 + o2 = f and o3 + o4 = Sum|terms| in double-double
+template<class G>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfin, int prec,
+      double *o1, double *o2, double *o3, double *o4){
+    constexpr int D = G::D;
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x; if(i >= n) return;
+    double sf[D];
+#pragma unroll
+    for(int d = 0; d < D; d++) sf[d] = sfin[i*D + d];
+    (void)pts;
+    if(prec == 0){ double ab = 0; double f = G::template f<double>(sf, &ab, 1); o1[i] = f; o2[i] = ab; }
+    else {
+        sfx_dd sq[D];
+#pragma unroll
+        for(int d = 0; d < D; d++) sq[d] = sfx_dd(sf[d]);
+        sfx_dd ab; sfx_dd f = G::template f<sfx_dd>(sq, &ab, 1);
+        o1[i] = f.hi; o2[i] = f.lo; o3[i] = ab.hi; o4[i] = ab.lo;
+    }
+}
+
+}  // namespace sfx
+
+// host-side launchers of one graph, reached through the registry (sfx_registry.h)
+struct sfx_graph_entry {
+    const char *name, *volkov; int L, D, weight;
+    hipError_t (*eval)(hipStream_t s, int cells, const sfx_cell_t *pts, long n, int st, const sfx_eval_params_t *P,
+                       double *v, uint8_t *fl, int block);
+    hipError_t (*dd)(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, const uint32_t *cnt,
+                     long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
+    hipError_t (*raw)(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2, double *o3, double *o4);
+    const void *kfn[4];        // the kernels (eval nodes, eval cells, dd nodes, dd cells), for attribute queries
+};
+
+// the host launchers exist only in the host compilation pass; the device pass sees just the kernels
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
+template __global__ void sfx::k_eval<G, false>(const sfx_cell_t*, long, int, sfx_eval_params_t, double*, uint8_t*); \
+template __global__ void sfx::k_eval<G, true>(const sfx_cell_t*, long, int, sfx_eval_params_t, double*, uint8_t*);  \
+template __global__ void sfx::k_dd<G, false>(const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*); \
+template __global__ void sfx::k_dd<G, true>(const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*);  \
+template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, int, double*, double*, double*, double*);
+#else
+#define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
+static hipError_t sfx_eval_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, long n, int st,           \
+                                  const sfx_eval_params_t *P, double *v, uint8_t *fl, int block){            \
+    if(n <= 0) return hipSuccess;                                                                            \
+    (void)block;                                                                                             \
+    unsigned nb = (unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK);                                                 \
+    if(cells) hipLaunchKernelGGL((sfx::k_eval<G, true>), dim3(nb), dim3(SFX_BLOCK), 0, s, pts, n, st, *P, v, fl); \
+    else hipLaunchKernelGGL((sfx::k_eval<G, false>), dim3(nb), dim3(SFX_BLOCK), 0, s, pts, n, st, *P, v, fl);   \
+    return hipGetLastError();                                                                                \
+}                                                                                                            \
+static hipError_t sfx_dd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, \
+                                const uint32_t *cnt, long maxcnt, const sfx_eval_params_t *P, double *v,     \
+                                double *e2, uint8_t *nd){                                                    \
+    if(maxcnt <= 0) return hipSuccess;                                                                       \
+    long nb = (maxcnt + SFX_BLOCK - 1)/SFX_BLOCK; if(nb > 4096) nb = 4096;                                   \
+    if(cells) hipLaunchKernelGGL((sfx::k_dd<G, true>), dim3((unsigned)nb), dim3(SFX_BLOCK), 0, s, pts, st, idx, cnt, *P, v, e2, nd); \
+    else hipLaunchKernelGGL((sfx::k_dd<G, false>), dim3((unsigned)nb), dim3(SFX_BLOCK), 0, s, pts, st, idx, cnt, *P, v, e2, nd);   \
+    return hipGetLastError();                                                                                \
+}                                                                                                            \
+static hipError_t sfx_raw_##NAME(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2,  \
+                                 double *o3, double *o4){                                                    \
+    if(n <= 0) return hipSuccess;                                                                            \
+    hipLaunchKernelGGL((sfx::k_raw<G>), dim3((unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK)), dim3(SFX_BLOCK), 0, s, \
+                       (const sfx_cell_t*)nullptr, n, sf, prec, o1, o2, o3, o4);                             \
+    return hipGetLastError();                                                                                \
+}                                                                                                            \
+extern "C" const sfx_graph_entry SFX_E_##NAME = {#NAME, VOLKOV, L, D, W,                                      \
+    sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME,                                                          \
+    {(const void*)&sfx::k_eval<G, false>, (const void*)&sfx::k_eval<G, true>,                                \
+     (const void*)&sfx::k_dd<G, false>, (const void*)&sfx::k_dd<G, true>}};
+#endif

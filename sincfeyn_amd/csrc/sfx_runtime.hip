@@ -1,0 +1,435 @@
+// sfx_runtime.hip -- device state, graph registry, the graph-independent device kernels (ordered
+// compaction, deterministic double-double reductions of a refinement block) and the per-point C ABI.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include "sfx_device.hpp"
+
+namespace sfx {
+
+static thread_local char g_err[512];
+
+void set_error(const char *fmt, ...){
+    va_list ap; va_start(ap, fmt); vsnprintf(g_err, sizeof g_err, fmt, ap); va_end(ap);
+}
+int hip_fail(hipError_t e, const char *what, const char *file, int line){
+    set_error("%s:%d: %s: %s", file, line, what, hipGetErrorString(e));
+    return SFX_EDEVICE;
+}
+Lib &lib(){ static Lib L; return L; }
+int require_init(){
+    if(!lib().init){ set_error("sfx_init has not been called"); return SFX_ENOTINIT; }
+    return SFX_OK;
+}
+
+int DevBuf::reserve(size_t bytes){
+    if(bytes <= cap) return SFX_OK;
+    size_t want = cap ? cap : 4096;
+    while(want < bytes) want *= 2;
+    if(p){ hipError_t e = hipFree(p); p = nullptr; cap = 0; if(e != hipSuccess) return hip_fail(e, "hipFree", __FILE__, __LINE__); }
+    hipError_t e = hipMalloc(&p, want);
+    if(e != hipSuccess){ p = nullptr; set_error("hipMalloc(%zu bytes): %s", want, hipGetErrorString(e)); return SFX_ENOMEM; }
+    cap = want;
+    return SFX_OK;
+}
+void DevBuf::release(){ if(p) (void)hipFree(p); p = nullptr; cap = 0; }
+
+// ---------------------------------------------------------------------------------------------------
+// double-double helpers for the reductions (the sum of two doubles exactly, then renormalised)
+__device__ __forceinline__ void dacc_add(double &hi, double &lo, double x){
+    double s = hi + x, bb = s - hi, e = (hi - (s - bb)) + (x - bb);
+    lo += e; double t = s + lo; lo = lo - (t - s); hi = t;
+}
+__device__ __forceinline__ void dacc_addd(double &hi, double &lo, double xh, double xl){
+    double s = hi + xh, bb = s - hi, e = (hi - (s - bb)) + (xh - bb);
+    lo += e + xl; double t = s + lo; lo = lo - (t - s); hi = t;
+}
+__device__ __forceinline__ double shfl_down(double x, int d){ return __shfl_down(x, d, 64); }
+
+// block-wide exclusive prefix of a per-thread predicate (blockDim = 256 = 4 waves); returns the rank of
+// this thread's item among the set ones, *tot = number set in the block
+__device__ __forceinline__ unsigned block_rank(bool pred, unsigned *sh, unsigned *tot){
+    unsigned long long m = __ballot(pred);
+    int lane = __lane_id(), wid = threadIdx.x >> 6;
+    unsigned r = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if(lane == 0) sh[wid] = (unsigned)__popcll(m);
+    __syncthreads();
+    unsigned base = 0, all = 0;
+    for(int w = 0; w < (int)(blockDim.x >> 6); w++){ unsigned c = sh[w]; if(w < wid) base += c; all += c; }
+    __syncthreads();
+    *tot = all;
+    return base + r;
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_count_flags(const uint8_t *fl, long n, uint8_t mask, uint32_t *blk){
+    long base = (long)blockIdx.x*TILE;
+    unsigned c = 0;
+    for(int j = 0; j < TILE_ITEMS; j++){ long i = base + j*TILE_THREADS + threadIdx.x; if(i < n && (fl[i] & mask)) c++; }
+    for(int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
+    __shared__ unsigned sh[TILE_THREADS/64];
+    if(__lane_id() == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0){ unsigned t = 0; for(int w = 0; w < TILE_THREADS/64; w++) t += sh[w]; blk[blockIdx.x] = t; }
+}
+
+// exclusive scan of cnt[0..nb-1] into off[], total into *total (+ *base if base != null); one block
+__global__ void __launch_bounds__(1024) k_scan_u32(const uint32_t *cnt, long nb, uint32_t *off, uint32_t *total,
+                                                   const unsigned long long *base){
+    __shared__ unsigned long long sh[1024];
+    long per = (nb + blockDim.x - 1)/blockDim.x, b0 = threadIdx.x*per, b1 = b0 + per < nb ? b0 + per : nb;
+    unsigned long long s = 0;
+    for(long b = b0; b < b1; b++) s += cnt[b];
+    sh[threadIdx.x] = s; __syncthreads();
+    for(int d = 1; d < (int)blockDim.x; d <<= 1){
+        unsigned long long x = threadIdx.x >= (unsigned)d ? sh[threadIdx.x - d] : 0; __syncthreads();
+        sh[threadIdx.x] += x; __syncthreads();
+    }
+    unsigned long long ex = sh[threadIdx.x] - s + (base ? *base : 0);
+    for(long b = b0; b < b1; b++){ off[b] = (uint32_t)ex; ex += cnt[b]; }
+    if(threadIdx.x == blockDim.x - 1 && total) *total = (uint32_t)sh[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_scatter_flags(const uint8_t *fl, long n, uint8_t mask, const uint32_t *off, uint32_t *out){
+    __shared__ unsigned sh[TILE_THREADS/64];
+    long base = (long)blockIdx.x*TILE;
+    unsigned run = off[blockIdx.x];
+    for(int j = 0; j < TILE_ITEMS; j++){
+        long i = base + j*TILE_THREADS + threadIdx.x;
+        bool p = i < n && (fl[i] & mask);
+        unsigned tot, r = block_rank(p, sh, &tot);
+        if(p) out[run + r] = (uint32_t)i;
+        run += tot;
+    }
+}
+
+hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask, uint32_t *blk, uint32_t *out, uint32_t *count){
+    long nb = (n + TILE - 1)/TILE;
+    if(nb == 0) return hipMemsetAsync(count, 0, sizeof(uint32_t), s);
+    hipLaunchKernelGGL(k_count_flags, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, fl, n, mask, blk);
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, s, blk, nb, blk + nb, count, (const unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_scatter_flags, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, fl, n, mask, blk + nb, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------
+// refinement block: partial sums per tile
+__device__ __forceinline__ int tier(double f, double t){ return (f >= 10*t) + (f >= 100*t); }
+
+__global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
+    long base = (long)blockIdx.x*TILE;
+    double s[2] = {0, 0}, ab[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0}, dr[2] = {0, 0};
+    unsigned nv = 0, nsk = 0, nk = 0;
+    for(int j = 0; j < TILE_ITEMS; j++){
+        long i = base + j*TILE_THREADS + threadIdx.x;
+        if(i >= a.n) break;
+        uint8_t f = a.fl[i];
+        if(!(f & SFX_FL_VALID)) continue;
+        double v = a.v[i], x = v*a.pw, av = fabs(v);
+        unsigned cfl = a.cells[i >> a.D].fl;
+        nv++; nsk += (a.abskip && (cfl & 1u)) ? 1u : 0u;
+        dacc_add(s[0], s[1], x); dacc_add(ab[0], ab[1], av);
+        if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(b1[0], b1[1], x); if(te >= 2) dacc_add(b2[0], b2[1], x); }
+        else if(av >= a.thr) nk++;
+        else dacc_add(dr[0], dr[1], av*a.pw);
+    }
+    // fixed reduction tree: wave shuffles, then the 4 waves in order
+    for(int d = 32; d > 0; d >>= 1){
+        dacc_addd(s[0], s[1], shfl_down(s[0], d), shfl_down(s[1], d));
+        dacc_addd(ab[0], ab[1], shfl_down(ab[0], d), shfl_down(ab[1], d));
+        dacc_addd(b1[0], b1[1], shfl_down(b1[0], d), shfl_down(b1[1], d));
+        dacc_addd(b2[0], b2[1], shfl_down(b2[0], d), shfl_down(b2[1], d));
+        dacc_addd(dr[0], dr[1], shfl_down(dr[0], d), shfl_down(dr[1], d));
+        nv += __shfl_down(nv, d, 64); nsk += __shfl_down(nsk, d, 64); nk += __shfl_down(nk, d, 64);
+    }
+    __shared__ LevelPart sh[TILE_THREADS/64];
+    if(__lane_id() == 0){
+        LevelPart &p = sh[threadIdx.x >> 6];
+        p.s[0] = s[0]; p.s[1] = s[1]; p.a[0] = ab[0]; p.a[1] = ab[1]; p.b1[0] = b1[0]; p.b1[1] = b1[1];
+        p.b2[0] = b2[0]; p.b2[1] = b2[1]; p.drop[0] = dr[0]; p.drop[1] = dr[1]; p.nv = nv; p.nskip = nsk; p.nkeep = nk;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        LevelPart t = sh[0];
+        for(int w = 1; w < TILE_THREADS/64; w++){
+            const LevelPart &q = sh[w];
+            dacc_addd(t.s[0], t.s[1], q.s[0], q.s[1]); dacc_addd(t.a[0], t.a[1], q.a[0], q.a[1]);
+            dacc_addd(t.b1[0], t.b1[1], q.b1[0], q.b1[1]); dacc_addd(t.b2[0], t.b2[1], q.b2[0], q.b2[1]);
+            dacc_addd(t.drop[0], t.drop[1], q.drop[0], q.drop[1]);
+            t.nv += q.nv; t.nskip += q.nskip; t.nkeep += q.nkeep;
+        }
+        a.part[blockIdx.x] = t;
+        a.keepoff[blockIdx.x] = t.nkeep;
+    }
+}
+
+// one block of CB threads: the tiles' partials in tile order, the fallback points' e^2 and unresolved
+// marks in list order, the kept counts scanned into offsets; everything added to the level accumulator
+constexpr int CB = 256;
+__global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
+    __shared__ double sh[6][2][CB];
+    __shared__ unsigned long long shc[4][CB];
+    long per = (nb + CB - 1)/CB, b0 = threadIdx.x*per, b1 = b0 + per < nb ? b0 + per : nb;
+    double v[6][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    unsigned long long nv = 0, nsk = 0, nk = 0, nd = 0;
+    for(long b = b0; b < b1; b++){
+        const LevelPart &p = a.part[b];
+        dacc_addd(v[0][0], v[0][1], p.s[0], p.s[1]); dacc_addd(v[1][0], v[1][1], p.a[0], p.a[1]);
+        dacc_addd(v[2][0], v[2][1], p.b1[0], p.b1[1]); dacc_addd(v[3][0], v[3][1], p.b2[0], p.b2[1]);
+        dacc_addd(v[4][0], v[4][1], p.drop[0], p.drop[1]);
+        nv += p.nv; nsk += p.nskip; nk += p.nkeep;
+    }
+    // fallback points, in list order
+    long m = a.ddcnt ? (long)*a.ddcnt : 0, perq = (m + CB - 1)/CB, q0 = threadIdx.x*perq, q1 = q0 + perq < m ? q0 + perq : m;
+    for(long j = q0; j < q1; j++){ dacc_add(v[5][0], v[5][1], a.e2[j]); nd += a.nd[j]; }
+    // exclusive scan of the kept counts over the threads' tile ranges, offsets from acc->nkept
+    shc[0][threadIdx.x] = nk; __syncthreads();
+    for(int d = 1; d < CB; d <<= 1){
+        unsigned long long x = threadIdx.x >= (unsigned)d ? shc[0][threadIdx.x - d] : 0; __syncthreads();
+        shc[0][threadIdx.x] += x; __syncthreads();
+    }
+    {
+        unsigned long long ex = shc[0][threadIdx.x] - nk + a.acc->nkept;
+        for(long b = b0; b < b1; b++){ a.keepoff[b] = (uint32_t)ex; ex += a.part[b].nkeep; }
+    }
+    __syncthreads();
+    for(int k = 0; k < 6; k++){ sh[k][0][threadIdx.x] = v[k][0]; sh[k][1][threadIdx.x] = v[k][1]; }
+    shc[0][threadIdx.x] = nv; shc[1][threadIdx.x] = nsk; shc[2][threadIdx.x] = nk; shc[3][threadIdx.x] = nd;
+    __syncthreads();
+    // fixed pairwise tree over the threads
+    for(int d = CB/2; d > 0; d >>= 1){
+        if(threadIdx.x < (unsigned)d){
+            int o = threadIdx.x + d;
+            for(int k = 0; k < 6; k++){
+                double h = sh[k][0][threadIdx.x], l = sh[k][1][threadIdx.x];
+                dacc_addd(h, l, sh[k][0][o], sh[k][1][o]);
+                sh[k][0][threadIdx.x] = h; sh[k][1][threadIdx.x] = l;
+            }
+            for(int k = 0; k < 4; k++) shc[k][threadIdx.x] += shc[k][o];
+        }
+        __syncthreads();
+    }
+    if(threadIdx.x == 0){
+        LevelAcc *A = a.acc;
+        dacc_addd(A->s[0], A->s[1], sh[0][0][0], sh[0][1][0]);
+        dacc_addd(A->a[0], A->a[1], sh[1][0][0], sh[1][1][0]);
+        dacc_addd(A->b1[0], A->b1[1], sh[2][0][0], sh[2][1][0]);
+        dacc_addd(A->b2[0], A->b2[1], sh[3][0][0], sh[3][1][0]);
+        dacc_addd(A->drop[0], A->drop[1], sh[4][0][0], sh[4][1][0]);
+        dacc_addd(A->u[0], A->u[1], sh[5][0][0], sh[5][1][0]);
+        A->nv += shc[0][0]; A->nskip += shc[1][0]; A->nkept += shc[2][0]; A->nd += shc[3][0]; A->nq += (unsigned long long)m;
+    }
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_level_scatter(LevelArgs a){
+    __shared__ unsigned sh[TILE_THREADS/64];
+    long base = (long)blockIdx.x*TILE;
+    unsigned run = a.keepoff[blockIdx.x];
+    int D = a.D, msk = (1 << D) - 1;
+    for(int j = 0; j < TILE_ITEMS; j++){
+        long i = base + j*TILE_THREADS + threadIdx.x;
+        bool p = false; double av = 0;
+        if(i < a.n && (a.fl[i] & SFX_FL_VALID)){ av = fabs(a.v[i]); p = av >= a.thr; }
+        unsigned tot, r = block_rank(p, sh, &tot);
+        if(p && (long)(run + r) < a.nxt_cap){
+            const sfx_cell_t cell = a.cells[i >> D];
+            int b = (int)(i & msk);
+            sfx_cell_t o;
+            for(int d = 0; d < SFX_MAXD; d++) o.k[d] = d < D ? (int8_t)(cell.k[d] - (((b >> d) & 1) ? a.st : 0)) : 0;
+            o.pad = 0;
+            int psk = a.abskip && (cell.fl & 1u), near = (a.fl[i] & SFX_FL_NEAR) != 0;
+            int te = cell.fl >> 1, tv = tier(av, a.thr);
+            o.fl = (uint8_t)((a.abskip && (psk || !near) ? 1 : 0) | ((te < tv ? te : tv) << 1));
+            a.nxt[run + r] = o;
+        }
+        run += tot;
+    }
+}
+
+hipError_t level_reduce(hipStream_t s, const LevelArgs &a){
+    long nb = (a.n + TILE - 1)/TILE;
+    if(nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_level_partials, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_level_combine, dim3(1), dim3(CB), 0, s, a, nb);
+    if(!a.last) hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace sfx
+
+using namespace sfx;
+
+// ---------------------------------------------------------------------------------------------------
+// C ABI: library and registry
+extern "C" const char *sfx_last_error(void){ return g_err; }
+
+extern "C" int sfx_init(int device){
+    Lib &L = lib();
+    if(L.init){
+        if(device == L.device) return SFX_OK;
+        set_error("sfx_init: already initialised on device %d", L.device); return SFX_EINVAL;
+    }
+    int n = 0;
+    SFX_HIP(hipGetDeviceCount(&n));
+    if(device < 0 || device >= n){ set_error("sfx_init: device %d of %d", device, n); return SFX_EINVAL; }
+    SFX_HIP(hipSetDevice(device));
+    SFX_HIP(hipGetDeviceProperties(&L.prop, device));
+    SFX_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    L.device = device; L.init = true;
+    char b[512];
+    snprintf(b, sizeof b, "%s (%s), %d CUs, %.0f MHz, %.1f GB HBM, %d graphs", L.prop.name, L.prop.gcnArchName,
+             L.prop.multiProcessorCount, L.prop.clockRate/1e3, L.prop.totalGlobalMem/1e9, SFX_NGRAPHS);
+    L.info = b;
+    return SFX_OK;
+}
+
+void sfx_release_job_buffers(void);
+extern "C" void sfx_shutdown(void){
+    Lib &L = lib();
+    if(!L.init) return;
+    sfx_release_job_buffers();
+    (void)hipStreamSynchronize(L.stream);
+    (void)hipStreamDestroy(L.stream);
+    L.stream = nullptr; L.init = false; L.device = -1;
+}
+
+extern "C" const char *sfx_info(void){ return lib().init ? lib().info.c_str() : "sfx: not initialised"; }
+
+extern "C" void sfx_opts_default(sfx_opts_t *o){
+    memset(o, 0, sizeof *o);
+    o->abserr0 = 1e-15; o->abrel = 1e-3; o->nearf = 1e-3; o->ddslack = 1; o->abskip = 1; o->dropunres = 1;
+    o->keepbt = 0; o->btmargin = 2; o->block_points = 0; o->verbose = 0;
+}
+
+extern "C" int sfx_graph_count(void){ return SFX_NGRAPHS; }
+
+extern "C" int sfx_graph_info(int i, sfx_graph_info_t *out){
+    if(i < 0 || i >= SFX_NGRAPHS || !out){ set_error("sfx_graph_info: index %d of %d", i, SFX_NGRAPHS); return SFX_EINVAL; }
+    const sfx_graph_entry *g = SFX_GRAPHS[i];
+    memset(out, 0, sizeof *out);
+    snprintf(out->name, sizeof out->name, "%s", g->name);
+    snprintf(out->volkov, sizeof out->volkov, "%s", g->volkov);
+    out->L = g->L; out->D = g->D; out->weight = g->weight;
+    if(lib().init){
+        hipFuncAttributes fa;
+        if(hipFuncGetAttributes(&fa, g->kfn[1]) == hipSuccess){
+            out->vgprs = fa.numRegs; out->scratch_bytes = (int)fa.localSizeBytes;
+            int occ = 0;
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->kfn[1], SFX_BLOCK, 0) == hipSuccess) out->occupancy = occ;
+        }
+    }
+    return SFX_OK;
+}
+
+extern "C" int sfx_graph_find(const char *s){
+    if(!s) return SFX_EINVAL;
+    for(int i = 0; i < SFX_NGRAPHS; i++) if(!strcmp(SFX_GRAPHS[i]->name, s) || !strcmp(SFX_GRAPHS[i]->volkov, s)) return i;
+    set_error("no graph %s in this build", s);
+    return SFX_EINVAL;
+}
+
+// graphs.c:17-29
+extern "C" int sfx_graph_selected(int i, const char *s){
+    if(i < 0 || i >= SFX_NGRAPHS || !s) return 0;
+    const sfx_graph_entry *g = SFX_GRAPHS[i];
+    if(!strcmp(s, g->name) || !strcmp(s, g->volkov)) return 1;
+    if(!strcmp(s, "all2")) return g->L == 2;
+    if(!strcmp(s, "all3")) return g->L == 3 && strncmp(g->name, "w3_", 3);
+    if(!strcmp(s, "all4")) return g->L == 4;
+    if(!strcmp(s, "q3"))   return !strncmp(g->name, "g3_", 3) && strcmp(g->name, "g3_lbl");
+    if(!strcmp(s, "vp3"))  return !strncmp(g->name, "v3_", 3);
+    if(!strcmp(s, "lbl"))  return !strcmp(g->name, "g3_lbl");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// per-point primitives
+static int check_graph(int graph){
+    if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("graph index %d out of range (%d graphs)", graph, SFX_NGRAPHS); return SFX_EINVAL; }
+    return SFX_OK;
+}
+
+extern "C" int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out){
+    int rc;
+    if((rc = require_init()) || (rc = check_graph(graph))) return rc;
+    if(n < 0 || (n && (!sf || !out)) || (prec != 0 && prec != 1)){ set_error("sfx_eval_raw: bad arguments"); return SFX_EINVAL; }
+    if(n == 0) return SFX_OK;
+    const sfx_graph_entry *g = SFX_GRAPHS[graph];
+    hipStream_t s = lib().stream;
+    int D = g->D, no = prec ? 4 : 2;
+    DevBuf dsf, dout;
+    if((rc = dsf.reserve((size_t)n*D*sizeof(double))) || (rc = dout.reserve((size_t)n*4*sizeof(double)))){ dsf.release(); dout.release(); return rc; }
+    double *o = dout.as<double>();
+    hipError_t e = hipMemcpyAsync(dsf.p, sf, (size_t)n*D*sizeof(double), hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = g->raw(s, n, dsf.as<double>(), prec, o, o + n, o + 2*n, o + 3*n);
+    std::vector<double> h((size_t)n*4);
+    if(e == hipSuccess) e = hipMemcpyAsync(h.data(), o, (size_t)n*4*sizeof(double), hipMemcpyDeviceToHost, s);
+    if(e == hipSuccess) e = hipStreamSynchronize(s);
+    dsf.release(); dout.release();
+    if(e != hipSuccess) return hip_fail(e, "sfx_eval_raw", __FILE__, __LINE__);
+    for(long i = 0; i < n; i++) for(int j = 0; j < no; j++) out[i*no + j] = h[(size_t)j*n + i];
+    return SFX_OK;
+}
+
+// f w and Sum|terms| w at nodes (the reference GPU kernel's mode 0/1 outputs): s and w on the host as
+// lattice_node does, f and ab on the device in FP64, the products in FP64
+static int eval_weighted(int graph, double h, int K, long n, const int *kk, double *fw, double *abw){
+    const sfx_graph_entry *g = SFX_GRAPHS[graph];
+    int D = g->D;
+    std::vector<double> EX(2*K + 1);
+    for(int k = -K; k <= K; k++) EX[k + K] = exp(k*h);
+    std::vector<double> sf((size_t)n*D), w(n), out((size_t)n*2);
+    for(long i = 0; i < n; i++){
+        double ww = 1;
+        for(int d = 0; d < D; d++){ double s = EX[kk[i*D + d] + K]; sf[(size_t)i*D + d] = s; ww *= s*h; }
+        w[i] = ww;
+    }
+    int rc = sfx_eval_raw(graph, n, sf.data(), 0, out.data());
+    if(rc) return rc;
+    for(long i = 0; i < n; i++){ fw[i] = out[2*i]*w[i]; abw[i] = out[2*i + 1]*w[i]; }
+    return SFX_OK;
+}
+
+extern "C" int sfx_eval_nodes(int graph, double h, int K, long n, const int16_t *k, const uint8_t *skip,
+                              double *fw, double *abw){
+    int rc;
+    if((rc = require_init()) || (rc = check_graph(graph))) return rc;
+    (void)skip;
+    if(n < 0 || K < 1 || K > 127 || !(h > 0) || (n && (!k || !fw || !abw))){ set_error("sfx_eval_nodes: bad arguments"); return SFX_EINVAL; }
+    int D = SFX_GRAPHS[graph]->D;
+    std::vector<int> kk((size_t)n*D);
+    for(long i = 0; i < n*D; i++){
+        if(k[i] < -K || k[i] > K){ set_error("sfx_eval_nodes: coordinate %d outside |k| <= %d", k[i], K); return SFX_EINVAL; }
+        kk[i] = k[i];
+    }
+    return n ? eval_weighted(graph, h, K, n, kk.data(), fw, abw) : SFX_OK;
+}
+
+extern "C" int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t *cells, const uint8_t *skip, int st,
+                              double *fw, double *abw){
+    int rc;
+    if((rc = require_init()) || (rc = check_graph(graph))) return rc;
+    (void)skip;
+    int D = SFX_GRAPHS[graph]->D;
+    if(nc < 0 || K < 1 || K > 127 || st < 1 || !(h > 0) || (nc && (!cells || !fw || !abw))){ set_error("sfx_eval_cells: bad arguments"); return SFX_EINVAL; }
+    long n = nc << D;
+    std::vector<int> kk; std::vector<long> at; kk.reserve((size_t)n*D);
+    for(long c = 0; c < nc; c++){
+        for(int d = 0; d < D; d++) if(cells[c*D + d] < -K || cells[c*D + d] > K){ set_error("sfx_eval_cells: coordinate outside the box"); return SFX_EINVAL; }
+        for(long b = 0; b < (1L << D); b++){
+            bool ok = true; int t[SFX_MAXD];
+            for(int d = 0; d < D; d++){ t[d] = cells[c*D + d] - (((b >> d) & 1) ? st : 0); if(t[d] < -K) ok = false; }
+            long i = (c << D) | b;
+            if(!ok){ fw[i] = 0; abw[i] = 0; continue; }
+            for(int d = 0; d < D; d++) kk.push_back(t[d]);
+            at.push_back(i);
+        }
+    }
+    long m = (long)at.size();
+    std::vector<double> f(m), a(m);
+    if(m && (rc = eval_weighted(graph, h, K, m, kk.data(), f.data(), a.data()))) return rc;
+    for(long j = 0; j < m; j++){ fw[at[j]] = f[j]; abw[at[j]] = a[j]; }
+    return SFX_OK;
+}

@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""hipgen.py -- HIP emitter for sincFeyn's generated integrand kernels (gfx950).
+
+Input: one generated kernel file of the reference, kernels/<class>/<graph>.c.  Each holds the integrand
+twice, once written against REAL = double (with SIMD helpers for the host) and once against REAL = QREAL,
+the high-precision type (reference driver/common.h:5-22, kernels/g2/g2_00.c:231-460).  The QREAL half is
+a complete scalar copy of the integrand: chunk functions bk_/ab_ (the forest blocks of large kernels), the
+point function pt_, helpers vp_/p_/nd_ and the Sinc wrapper f_, all plain + - * / on REAL.  The emitter
+turns that half into ONE set of C++ function templates over REAL, which the runtime instantiates twice on
+the device: REAL = double for the bulk of the points and REAL = sfx_dd for the fallback.  So both device
+paths come from the same text, and the double path is the arithmetic of the reference's own GPU build
+(which compiles the same half as double, reference tools/gpugen.py:20-29).
+
+Transforms (text level; the kernels are regular machine-written C):
+  * `static REAL FN(name)(params){`    -> `template<class REAL> SFX_FN REAL name(params, const int SQ_AB){`
+  * `static void FN(bk_x)(const REAL *F, REAL *T){` (chunk) -> `template<class REAL> SFX_CHUNK void bk_x(...)`
+  * every call `FN(name)(args)`         -> `name<REAL>(args, SQ_AB)`  (chunk calls get no SQ_AB)
+  * the Gauss-Legendre tables at file top -> `__constant__` arrays
+  * SQ_AB (the reference's thread-local "compute the |.| sums" flag, common.h:24-26) becomes a parameter;
+    the runtime passes a compile-time constant, so with 0 the companion sums are dead code.
+The graph descriptor line `const graph_t G_<name>={...}` (common.h:38) gives name, Volkov string, loops L,
+dimension D and mirror weight.  The emitter also counts the FP64 + - * / of every function body (a
+static count, loops expanded by their literal bounds where the code has them), for the roofline report.
+
+    python3 -m sincfeyn_amd.emit.hipgen KERNEL.c OUT_DIR     -> OUT_DIR/k_<name>.hip and k_<name>.json
+"""
+import json
+import os
+import re
+import sys
+
+DESC_RE = re.compile(r'^const graph_t G_(\w+)=\{"([^"]*)","([^"]*)",(\d+),(\d+),(\d+),(\d+),(\d+),', re.M)
+GL_RE = re.compile(r'^static const double (GL[A-Z]*X_\w+)\[(\d+)\]=\{([^}]*)\}, (GL[A-Z]*W_\w+)\[(\d+)\]=\{([^}]*)\};', re.M)
+DEF_RE = re.compile(r'^static (REAL|void) FN\((\w+)\)\(', re.M)
+
+
+class EmitError(Exception):
+    pass
+
+
+def _match_paren(s, i):
+    """index of the ')' closing the '(' at s[i]"""
+    depth = 0
+    for p in range(i, len(s)):
+        c = s[p]
+        if c == '(':
+            depth += 1
+        elif c == ')':
+            depth -= 1
+            if depth == 0:
+                return p
+    raise EmitError('unbalanced parentheses')
+
+
+def parse_kernel(text):
+    """(descriptor dict, GL tables, QREAL half text) of one generated kernel file"""
+    m = DESC_RE.search(text)
+    if not m:
+        raise EmitError('no graph_t descriptor')
+    if m.group(2) != m.group(1):
+        raise EmitError('descriptor G_%s names graph %s' % (m.group(1), m.group(2)))
+    desc = dict(name=m.group(1), volkov=m.group(3), L=int(m.group(4)), D=int(m.group(5)),
+                weight=int(m.group(6)), n=int(m.group(7)), fixed=int(m.group(8)))
+    start = text.find('#define REAL QREAL')
+    if start < 0:
+        raise EmitError('no QREAL half')
+    fn = text.find('#define FN(x) x##_q', start)
+    if fn < 0:
+        raise EmitError('no FN(x) x##_q after #define REAL QREAL')
+    body_start = fn + len('#define FN(x) x##_q')
+    end = text.find('#undef REAL', body_start)
+    if end < 0:
+        raise EmitError('QREAL half not closed by #undef REAL')
+    tables = [(g[0], int(g[1]), g[2], g[3], int(g[4]), g[5]) for g in GL_RE.findall(text)]
+    return desc, tables, text[body_start:end]
+
+
+def _is_chunk(name):
+    return name.startswith('bk_') or name.startswith('ab_')
+
+
+def transform(body):
+    """QREAL half -> templated device code; returns (code, function names)"""
+    defs = DEF_RE.findall(body)
+    if not defs:
+        raise EmitError('no functions in the QREAL half')
+    names = [n for _, n in defs]
+    chunk = {n for n in names if _is_chunk(n)}
+    # calls first: every FN(name)( not preceded by "static REAL " / "static void "
+    out, pos = [], 0
+    for m in re.finditer(r'FN\((\w+)\)\(', body):
+        name = m.group(1)
+        if name not in names:
+            raise EmitError('call of an unknown function %s' % name)
+        head = body[max(0, m.start() - 12):m.start()]
+        if re.search(r'static (REAL|void) $', head):
+            continue
+        close = _match_paren(body, m.end() - 1)
+        out.append(body[pos:m.start()])
+        out.append('%s<REAL>(' % name)
+        args = body[m.end():close]
+        out.append(args)
+        if name not in chunk:
+            out.append(', SQ_AB')
+        pos = close
+    out.append(body[pos:])
+    body = ''.join(out)
+    # definitions
+    res, pos = [], 0
+    for m in re.finditer(r'^static (REAL|void) FN\((\w+)\)\(', body, re.M):
+        ret, name = m.group(1), m.group(2)
+        close = _match_paren(body, m.end() - 1)
+        params = body[m.end():close]
+        res.append(body[pos:m.start()])
+        if name in chunk:
+            params = params.replace('const REAL *F', 'const REAL *__restrict__ F').replace('REAL *T', 'REAL *__restrict__ T')
+            res.append('template<class REAL> SFX_CHUNK %s %s(%s' % (ret, name, params))
+        else:
+            res.append('template<class REAL> SFX_FN %s %s(%s, const int SQ_AB' % (ret, name, params))
+        pos = close
+    res.append(body[pos:])
+    code = ''.join(res)
+    if re.search(r'\bstatic\b', code):
+        raise EmitError('unexpected static declaration left in the QREAL half')
+    if re.search(r'\bFN\(', code):
+        raise EmitError('FN( left after the transform')
+    return code, names
+
+
+_OPS_RE = re.compile(r'(?<![eE])[-+*/](?![=+\-])')
+
+
+def count_ops(body):
+    """static count of the binary + - * / in the statements of the QREAL half (compound assignments
+    count once; unary minus after '(' '=' ',' is not counted).  Informational: the roofline uses the
+    per-point counts measured by the tests, this is the emitter's own upper estimate."""
+    n = 0
+    for line in body.splitlines():
+        s = line.strip()
+        if not s or s.startswith('/*') or s.startswith('#') or s.startswith('for('):
+            continue
+        s = re.sub(r'\[[^\]]*\]', '[]', s)
+        s = re.sub(r'(^|[=(,?:])\s*-', r'\1', s)
+        n += len(_OPS_RE.findall(s)) + len(re.findall(r'[-+*/]=', s))
+    return n
+
+
+def emit(src_path, out_dir):
+    with open(src_path) as f:
+        text = f.read()
+    desc, tables, body = parse_kernel(text)
+    code, names = transform(body)
+    name = desc['name']
+    if 'f_' + name not in names:
+        raise EmitError('%s: no f_%s in the QREAL half' % (src_path, name))
+    lines = ['// generated by sincfeyn_amd/emit/hipgen.py from %s -- do not edit' % os.path.basename(src_path),
+             '// graph %s: %s, L=%d, D=%d, weight %d' % (name, desc['volkov'], desc['L'], desc['D'], desc['weight']),
+             '#include "sfx_integrand.hpp"',
+             'namespace sfxk_%s {' % name]
+    for xn, xl, xv, wn, wl, wv in tables:
+        lines.append('static __constant__ const double %s[%d] = {%s}, %s[%d] = {%s};' % (xn, xl, xv, wn, wl, wv))
+    lines += ['#define RABS(x) sfx_abs(x)', code.strip('\n'), '#undef RABS',
+              'struct Graph {',
+              '    static constexpr int D = %d;' % desc['D'],
+              '    template<class REAL> static __device__ __forceinline__ REAL f(const REAL *sf, REAL *ab, const int sqab)',
+              '    { return f_%s<REAL>(sf, ab, sqab); }' % name,
+              '};',
+              '}  // namespace sfxk_%s' % name,
+              'SFX_GRAPH_ENTRY(%s, sfxk_%s::Graph, "%s", %d, %d, %d)' % (
+                  name, name, desc['volkov'].replace('"', '\\"'), desc['L'], desc['D'], desc['weight']),
+              '']
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, 'k_%s.hip' % name)
+    tmp = out + '.tmp'
+    with open(tmp, 'w') as f:
+        f.write('\n'.join(lines))
+    os.replace(tmp, out)
+    meta = dict(desc, source=os.path.basename(src_path), functions=len(names),
+                chunk_functions=sum(1 for n in names if _is_chunk(n)), static_ops=count_ops(body))
+    with open(os.path.join(out_dir, 'k_%s.json' % name), 'w') as f:
+        json.dump(meta, f, indent=1)
+    return out, meta
+
+
+def main(argv):
+    if len(argv) != 3:
+        print(__doc__)
+        return 2
+    out, meta = emit(argv[1], argv[2])
+    print(out, meta['name'], 'D=%d' % meta['D'])
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main(sys.argv))
