@@ -121,6 +121,24 @@ int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, cons
  * the device with the cell lists resident in HBM. */
 int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res);
 
+/* ---- measurement ----
+ * With profiling on, every device kernel the library launches is bracketed by HIP events on the
+ * library's stream, and the elapsed times are accumulated per (graph, kernel kind) together with the
+ * points it processed (valid lattice points, and how many of them skipped the |.| sums). */
+#define SFX_K_LEVEL_EVAL 0     /* refinement: children of cells, FP64 + tests */
+#define SFX_K_LEVEL_DD   1     /* refinement: double-double recompute of the flagged children */
+#define SFX_K_LEVEL_SUM  2     /* refinement: compaction, block sums, next-level cells */
+#define SFX_K_NODES_EVAL 3     /* coarsest level: frontier nodes, FP64 + tests */
+#define SFX_K_NODES_DD   4     /* coarsest level: double-double recompute */
+typedef struct {
+    char graph[32]; int kind;
+    long launches; double ms; long points, skipped, dd_points;
+} sfx_prof_entry_t;
+int sfx_prof_enable(int on);           /* on: clears the table and starts recording; 0 stops */
+int sfx_prof_count(void);
+int sfx_prof_get(int i, sfx_prof_entry_t *out);
+int sfx_sync(void);                    /* waits for the library's stream */
+
 #ifdef __cplusplus
 }
 #endif

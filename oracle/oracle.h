@@ -37,7 +37,7 @@ typedef struct {
 typedef struct { char volkov[64]; int L, D, weight, has_fv; } orc_graph_t;
 
 void orc_opts_default(orc_opts_t *o);
-int  orc_open(const char *kernels_so);                       /* 0 on success */
+int  orc_open(const char *kernels_so, const char *prefix);   /* 0 on success; prefix "G_" or "Q_" */
 int  orc_graph(const char *name, orc_graph_t *out);
 /* the reference kernel at Sinc variables sf[i*D + d]: fd (double half) -> f[i], ab[i] */
 int  orc_eval_fd(const char *name, long n, const double *sf, double *f, double *ab);

@@ -16,6 +16,7 @@ LIB_ORACLE = os.path.join(HERE, '_build', 'liboracle.so')
 LIB_KERNELS = os.path.join(REF_DIR, 'libsqk_ref.so')
 LIB_KERNELS_NC = os.path.join(REF_DIR, 'libsqk_nc.so')
 SINCQED_REF = os.path.join(REF_DIR, 'sincqed_ref')
+SINCQED_REF_NC = os.path.join(REF_DIR, 'sincqed_ref_nc')
 
 
 class OrcOpts(C.Structure):
@@ -41,18 +42,27 @@ def available():
 
 
 class Oracle:
-    def __init__(self):
+    """nc=False: the kernels as the reference's default build compiles them (FMA contraction on, the CPU
+    path the reference reports); nc=True: the same kernels and driver semantics without contraction (the
+    floating-point semantics of the reference's GPU path, Makefile:188-191), which the device reproduces
+    point by point."""
+
+    def __init__(self, nc=False):
+        self.kset = (LIB_KERNELS_NC, b'Q_') if nc else (LIB_KERNELS, b'G_')
         self.lib = C.CDLL(LIB_ORACLE)
         L = self.lib
-        L.orc_open.argtypes = [C.c_char_p]
+        L.orc_open.argtypes = [C.c_char_p, C.c_char_p]
         L.orc_graph.argtypes = [C.c_char_p, C.POINTER(OrcGraph)]
         L.orc_eval_fd.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_eval_fq.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p]
         L.orc_hsum.argtypes = [C.c_char_p, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(OrcOpts), C.POINTER(OrcRes)]
         L.orc_opts_default.argtypes = [C.POINTER(OrcOpts)]
         L.orc_error.restype = C.c_char_p
-        if L.orc_open(LIB_KERNELS.encode()) != 0:
-            raise RuntimeError(L.orc_error().decode())
+        self._select()
+
+    def _select(self):
+        if self.lib.orc_open(self.kset[0].encode(), self.kset[1]) != 0:
+            raise RuntimeError(self.lib.orc_error().decode())
 
     def opts(self, **kw):
         o = OrcOpts()
@@ -62,12 +72,14 @@ class Oracle:
         return o
 
     def graph(self, name):
+        self._select()
         g = OrcGraph()
         if self.lib.orc_graph(name.encode(), C.byref(g)) != 0:
             raise KeyError(name)
         return dict(volkov=g.volkov.decode(), L=g.L, D=g.D, weight=g.weight, has_fv=g.has_fv)
 
     def eval_fd(self, name, sf):
+        self._select()
         import numpy as np
         sf = np.ascontiguousarray(sf, dtype=np.float64)
         n = sf.shape[0]
@@ -77,6 +89,7 @@ class Oracle:
         return f, ab
 
     def eval_fq(self, name, sf):
+        self._select()
         import numpy as np
         sf = np.ascontiguousarray(sf, dtype=np.float64)
         out = np.zeros((sf.shape[0], 4))
@@ -85,6 +98,7 @@ class Oracle:
         return out
 
     def hsum(self, name, h, N, eps, top, **kw):
+        self._select()
         r = OrcRes()
         o = self.opts(**kw)
         if self.lib.orc_hsum(name.encode(), h, N, eps, top, C.byref(o), C.byref(r)) != 0:
@@ -110,15 +124,16 @@ def default_top(h, topstep=3.0):
 _RLINE = re.compile(r'^#R (\S+) (.*)$')
 
 
-def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=600):
-    """the reference implementation (oracle/_ref/sincqed_ref) -> {graph: {field: value}}, wall seconds"""
+def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=600, nc=False):
+    """the reference implementation (oracle/_ref/sincqed_ref, or sincqed_ref_nc built without FP
+    contraction) -> ({graph: {field: value}}, wall seconds, stdout)"""
     e = dict(os.environ)
     e.update({'SINCQED_CKPT': '0', 'SINCQED_PROGRESS': '0', 'SINCQED_RESULTS': '-'})
     if threads:
         e['OMP_NUM_THREADS'] = str(threads)
     if env:
         e.update(env)
-    args = [SINCQED_REF, 'run', repr(float(h)), str(N), repr(float(eps))]
+    args = [SINCQED_REF_NC if nc else SINCQED_REF, 'run', repr(float(h)), str(N), repr(float(eps))]
     if top is not None:
         args.append(str(top))
     args += list(graphs)
@@ -137,3 +152,53 @@ def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=6
         elif line.startswith('# wall'):
             wall = float(line.split()[2])
     return out, wall, p.stdout
+
+
+class _Graph(C.Structure):
+    """graph_t (reference driver/common.h:38)"""
+    _fields_ = [('name', C.c_char_p), ('volkov', C.c_char_p), ('L', C.c_int), ('D', C.c_int), ('weight', C.c_int),
+                ('n', C.c_int), ('fixed', C.c_int), ('fd', C.c_void_p), ('fq', C.c_void_p), ('ptd', C.c_void_p),
+                ('ptq', C.c_void_p), ('fv', C.c_void_p)]
+
+
+class _DD(C.Structure):
+    _fields_ = [('hi', C.c_double), ('lo', C.c_double)]
+
+
+_FD = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+_FQ = C.CFUNCTYPE(_DD, C.POINTER(_DD), C.POINTER(_DD))
+
+
+class NcKernels:
+    """the reference kernels compiled with FP contraction off (oracle/_ref/libsqk_nc.so): G_<g>.fq is the
+    QREAL half evaluated in plain FP64 (the arithmetic of the device's FP64 path), Q_<g>.fq the QREAL half
+    in the reference's double-double (dd.h) -- the arithmetic of the device's fallback path."""
+
+    def __init__(self, path=LIB_KERNELS_NC):
+        self.lib = C.CDLL(path)
+
+    def fp64(self, name, sf):
+        import numpy as np
+        g = _Graph.in_dll(self.lib, 'G_' + name)
+        fq = _FD(g.fq)
+        sf = np.ascontiguousarray(sf, dtype=np.float64)
+        f = np.zeros(sf.shape[0]); ab = np.zeros(sf.shape[0]); a = C.c_double()
+        P = C.POINTER(C.c_double)
+        for i in range(sf.shape[0]):
+            f[i] = fq(sf[i].ctypes.data_as(P), C.byref(a))
+            ab[i] = a.value
+        return f, ab
+
+    def dd(self, name, sf):
+        import numpy as np
+        g = _Graph.in_dll(self.lib, 'Q_' + name)
+        fq = _FQ(g.fq)
+        sf = np.ascontiguousarray(sf, dtype=np.float64)
+        D = sf.shape[1]
+        out = np.zeros((sf.shape[0], 4)); arr = (_DD*D)(); ab = _DD()
+        for i in range(sf.shape[0]):
+            for d in range(D):
+                arr[d].hi = float(sf[i, d]); arr[d].lo = 0.0
+            v = fq(arr, C.byref(ab))
+            out[i] = (v.hi, v.lo, ab.hi, ab.lo)
+        return out

@@ -14,6 +14,9 @@
 #include "sq.h"
 #include <dlfcn.h>
 
+#ifndef SQ_PREFIX
+#define SQ_PREFIX "G_"          /* descriptor symbol prefix of the linked kernels */
+#endif
 #ifndef SQ_NAMES
 #error "SQ_NAMES (the kernel names, space separated) must be defined by the build"
 #endif
@@ -21,7 +24,7 @@
 static void load_graphs(void){
     char names[] = SQ_NAMES, *save = NULL;
     for(char *t = strtok_r(names, " ", &save); t; t = strtok_r(NULL, " ", &save)){
-        char sym[96]; snprintf(sym, sizeof sym, "G_%s", t);
+        char sym[96]; snprintf(sym, sizeof sym, "%s%s", SQ_PREFIX, t);
         const graph_t *g = (const graph_t*)dlsym(RTLD_DEFAULT, sym);
         if(!g){ fprintf(stderr, "sincqed_ref: no symbol %s\n", sym); exit(2); }
         add_graph(g);

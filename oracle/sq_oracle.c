@@ -18,7 +18,7 @@ typedef double (*fd_t)(const double*, double*);
 typedef ddv (*fq_t)(const ddv*, ddv*);
 typedef struct { const char *name, *volkov; int L, D, weight, n, fixed; void *fd, *fq, *ptd, *ptq, *fv; } graph_t;   /* common.h:38 */
 
-static void *LIB; static int (*SETAB)(int);
+static void *LIB; static int (*SETAB)(int); static char PREFIX[8] = "G_";
 static char ERR[256];
 const char *orc_error(void){ return ERR; }
 
@@ -27,15 +27,17 @@ void orc_opts_default(orc_opts_t *o){
     o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18;
 }
 
-int orc_open(const char *so){
-    if(LIB) return 0;
-    LIB = dlopen(so, RTLD_NOW | RTLD_LOCAL);
+/* prefix "G_": the kernels as the reference builds them (libsqk_ref.so); "Q_": built without FP
+ * contraction (libsqk_nc.so), fd the double half and fq the double-double half */
+int orc_open(const char *so, const char *prefix){
+    snprintf(PREFIX, sizeof PREFIX, "%s", prefix && *prefix ? prefix : "G_");
+    LIB = dlopen(so, RTLD_NOW | RTLD_LOCAL);          /* the loader keeps one handle per file: selecting again is cheap */
     if(!LIB){ snprintf(ERR, sizeof ERR, "dlopen %s: %s", so, dlerror()); return -1; }
     SETAB = (int(*)(int))dlsym(LIB, "sqk_set_ab");
     return 0;
 }
 static const graph_t *findg(const char *name){
-    char sym[96]; snprintf(sym, sizeof sym, "G_%s", name);
+    char sym[96]; snprintf(sym, sizeof sym, "%s%s", PREFIX, name);
     const graph_t *g = LIB ? (const graph_t*)dlsym(LIB, sym) : NULL;
     if(!g) snprintf(ERR, sizeof ERR, "no kernel %s", name);
     return g;

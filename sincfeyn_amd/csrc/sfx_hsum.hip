@@ -28,6 +28,30 @@ struct JobBufs {
 };
 static JobBufs &bufs(){ static JobBufs B; return B; }
 
+// ---- profiling: HIP events on the library stream around each kernel, accumulated per (graph, kind)
+struct Prof {
+    bool on = false;
+    std::vector<sfx_prof_entry_t> table;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+static Prof &prof(){ static Prof P; return P; }
+static int prof_events(){
+    Prof &P = prof();
+    for(int i = 0; i < 4; i++) if(!P.ev[i]) SFX_HIP(hipEventCreate(&P.ev[i]));
+    return SFX_OK;
+}
+static void prof_add(const char *graph, int kind, double ms, long points, long skipped, long dd){
+    Prof &P = prof();
+    for(sfx_prof_entry_t &e : P.table) if(e.kind == kind && !strcmp(e.graph, graph)){
+        e.launches++; e.ms += ms; e.points += points; e.skipped += skipped; e.dd_points += dd; return;
+    }
+    sfx_prof_entry_t e; memset(&e, 0, sizeof e);
+    snprintf(e.graph, sizeof e.graph, "%s", graph); e.kind = kind;
+    e.launches = 1; e.ms = ms; e.points = points; e.skipped = skipped; e.dd_points = dd;
+    P.table.push_back(e);
+}
+static double prof_ms(int a, int b){ float ms = 0; (void)hipEventElapsedTime(&ms, prof().ev[a], prof().ev[b]); return ms; }
+
 // Neumaier compensated sum (reference sum.c:18-20)
 struct Neu {
     double s = 0, c = 0;
@@ -83,10 +107,14 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
        (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double))) || (rc = B.nd.reserve(n))) return rc;
     hipStream_t s = J.s;
     SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
+    const bool pon = prof().on;
+    if(pon){ if((rc = prof_events())) return rc; SFX_HIP(hipEventRecord(prof().ev[0], s)); }
     SFX_HIP(J.g->eval(s, 0, B.pts.as<sfx_cell_t>(), n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+    if(pon) SFX_HIP(hipEventRecord(prof().ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     SFX_HIP(J.g->dd(s, 0, B.pts.as<sfx_cell_t>(), 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
                     B.v.as<double>(), B.e2.as<double>(), B.nd.as<uint8_t>()));
+    if(pon) SFX_HIP(hipEventRecord(prof().ev[2], s));
     uint32_t m = 0;
     std::vector<uint8_t> fl(n);
     SFX_HIP(hipMemcpyAsync(v, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
@@ -100,6 +128,7 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
         SFX_HIP(hipMemcpyAsync(idx.data(), B.ddidx.p, m*sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         SFX_HIP(hipStreamSynchronize(s));
     }
+    if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(1, 2), m, 0, m); }
     for(long i = 0; i < n; i++) flg[i] = fl[i] & (SFX_FL_NEAR | SFX_FL_DD);
     for(uint32_t j = 0; j < m; j++){ *u += e2[j]; if(ndv[j]){ (*nd)++; flg[idx[j]] |= SFX_PT_UNRES; } }
     *nq += m;
@@ -288,6 +317,17 @@ static void set_params(Job &J, double abserr){
     J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.EX = bufs().EX.as<double>();
 }
 
+// grow a device buffer keeping its first `used` bytes (DevBuf::reserve does not preserve contents)
+static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s){
+    if(want <= b.cap) return SFX_OK;
+    size_t c = b.cap ? b.cap : 4096; while(c < want) c *= 2;
+    DevBuf t; int rc = t.reserve(c); if(rc) return rc;
+    if(used) SFX_HIP(hipMemcpyAsync(t.p, b.p, used, hipMemcpyDeviceToDevice, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    b.release(); b = t;
+    return SFX_OK;
+}
+
 __global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0) memset(a, 0, sizeof *a); }
 
 // one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
@@ -303,27 +343,36 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
        (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
        (rc = B.acc.reserve(sizeof(LevelAcc)))) return rc;
     hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
-    long kept = 0;
+    long kept = 0, prof_nv = 0, prof_nsk = 0, prof_nq = 0;
     for(long c0 = 0; c0 < ncur; c0 += Bc){
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
+        const bool pon = prof().on;
+        if(pon){ if((rc = prof_events())) return rc; SFX_HIP(hipEventRecord(prof().ev[0], s)); }
         SFX_HIP(J.g->eval(s, 1, cells, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+        if(pon) SFX_HIP(hipEventRecord(prof().ev[1], s));
         SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
         SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P, B.v.as<double>(),
                         B.e2.as<double>(), B.nd.as<uint8_t>()));
-        if(!last){
-            // room for every child of this block in the next level's list (kept so far is known exactly)
-            if((rc = B.nxt.reserve((size_t)(kept + n)*sizeof(sfx_cell_t)))){
-                // a reallocation loses the kept cells: grow by copying through a temporary buffer
-                return rc;
-            }
-        }
+        if(pon) SFX_HIP(hipEventRecord(prof().ev[2], s));
+        // room for every child of this block in the next level's list (kept so far is known exactly)
+        if(!last && (rc = reserve_preserving(B.nxt, (size_t)kept*sizeof(sfx_cell_t), (size_t)(kept + n)*sizeof(sfx_cell_t), s))) return rc;
         LevelArgs a;
         a.v = B.v.as<double>(); a.fl = B.fl.as<uint8_t>(); a.cells = cells; a.n = n; a.D = D; a.st = st; a.last = last;
         a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nd = B.nd.as<uint8_t>();
         a.ddcnt = B.ddcnt.as<uint32_t>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
         a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<uint32_t>(); a.acc = B.acc.as<LevelAcc>();
         SFX_HIP(level_reduce(s, a));
+        if(pon){
+            SFX_HIP(hipEventRecord(prof().ev[3], s));
+            LevelAcc now_; SFX_HIP(hipMemcpyAsync(&now_, B.acc.p, sizeof now_, hipMemcpyDeviceToHost, s));
+            SFX_HIP(hipStreamSynchronize(s));
+            long dv = (long)now_.nv - prof_nv, dsk = (long)now_.nskip - prof_nsk, dq = (long)now_.nq - prof_nq;
+            prof_nv = (long)now_.nv; prof_nsk = (long)now_.nskip; prof_nq = (long)now_.nq;
+            prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(0, 1), dv, dsk, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(1, 2), dq, 0, dq);
+            prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(2, 3), n, 0, 0);
+        }
         if(!last){
             unsigned long long k = 0;
             SFX_HIP(hipMemcpyAsync(&k, (char*)B.acc.p + offsetof(LevelAcc, nkept), sizeof k, hipMemcpyDeviceToHost, s));
@@ -338,23 +387,58 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     return SFX_OK;
 }
 
-// keep the device list `nxt` when it must grow: DevBuf::reserve does not preserve contents, so the
-// next-level list is sized before the level starts to the bound ncur << D when that fits, else grown
-// with a copy (see refine_level_checked)
-static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s){
-    if(want <= b.cap) return SFX_OK;
-    DevBuf t; int rc = t.reserve(want); if(rc) return rc;
-    if(used) SFX_HIP(hipMemcpyAsync(t.p, b.p, used, hipMemcpyDeviceToDevice, s));
-    SFX_HIP(hipStreamSynchronize(s));
-    b.release(); b = t;
-    return SFX_OK;
-}
-
 }  // namespace sfx
 
 using namespace sfx;
 
 extern "C" void sfx_release_job_buffers(void){ bufs().release(); }
+
+extern "C" int sfx_prof_enable(int on){
+    Prof &P = prof();
+    if(on){ P.table.clear(); P.on = true; } else P.on = false;
+    return SFX_OK;
+}
+extern "C" int sfx_prof_count(void){ return (int)prof().table.size(); }
+extern "C" int sfx_prof_get(int i, sfx_prof_entry_t *out){
+    if(i < 0 || i >= (int)prof().table.size() || !out){ set_error("sfx_prof_get: index %d", i); return SFX_EINVAL; }
+    *out = prof().table[i];
+    return SFX_OK;
+}
+extern "C" int sfx_sync(void){
+    int rc; if((rc = require_init())) return rc;
+    SFX_HIP(hipStreamSynchronize(lib().stream));
+    return SFX_OK;
+}
+
+extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, const uint8_t *skip,
+                                 double abserr, const sfx_opts_t *opts, double *v, uint8_t *flags, long *nq){
+    int rc;
+    if((rc = require_init())) return rc;
+    if(graph < 0 || graph >= SFX_NGRAPHS || n < 0 || K < 1 || K > 127 || !(h > 0) || (n && (!k || !v || !flags))){
+        set_error("sfx_eval_finished: bad arguments"); return SFX_EINVAL;
+    }
+    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.K = K; J.eps = 0; J.top = 0; J.s = lib().stream;
+    if(opts) J.o = *opts; else sfx_opts_default(&J.o);
+    std::vector<sfx_cell_t> pts(n);
+    for(long i = 0; i < n; i++){
+        memset(&pts[i], 0, sizeof pts[i]);
+        for(int d = 0; d < J.D; d++){
+            int x = k[i*J.D + d];
+            if(x < -K || x > K){ set_error("sfx_eval_finished: coordinate %d outside |k| <= %d", x, K); return SFX_EINVAL; }
+            pts[i].k[d] = (int8_t)x;
+        }
+        pts[i].fl = skip && skip[i] ? 1 : 0;
+    }
+    std::vector<double> EX(2*K + 1);
+    for(int x = -K; x <= K; x++) EX[x + K] = exp(x*h);
+    if((rc = bufs().EX.reserve(EX.size()*sizeof(double)))) return rc;
+    SFX_HIP(hipMemcpyAsync(bufs().EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
+    set_params(J, abserr);
+    long q = 0, nd = 0; double u = 0;
+    if((rc = eval_nodes_finished(J, pts.data(), n, v, flags, &q, &u, &nd))) return rc;
+    if(nq) *nq = q;
+    return SFX_OK;
+}
 
 extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
     int rc;
@@ -401,11 +485,6 @@ extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const s
     res->esok = J.D < 12;
     for(int lev = top - 1; lev >= 0; lev--){
         long nnext = 0; LevelAcc A;
-        if(lev > 0){
-            // the next level's list: at most every child of every cell
-            size_t bound = (size_t)ncur << J.D;
-            if((rc = reserve_preserving(B.nxt, 0, bound*sizeof(sfx_cell_t), J.s))) return rc;
-        }
         if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))) return rc;
         double S = A.s[0] + A.s[1];
         res->prune += A.drop[0] + A.drop[1];

@@ -285,7 +285,7 @@ extern "C" int sfx_init(int device){
     return SFX_OK;
 }
 
-void sfx_release_job_buffers(void);
+extern "C" void sfx_release_job_buffers(void);
 extern "C" void sfx_shutdown(void){
     Lib &L = lib();
     if(!L.init) return;

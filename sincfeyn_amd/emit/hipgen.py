@@ -153,28 +153,35 @@ def emit(src_path, out_dir):
     name = desc['name']
     if 'f_' + name not in names:
         raise EmitError('%s: no f_%s in the QREAL half' % (src_path, name))
+    inc = ['// generated by sincfeyn_amd/emit/hipgen.py from %s -- do not edit' % os.path.basename(src_path),
+           '// the integrand of graph %s as function templates over REAL' % name]
+    for xn, xl, xv, wn, wl, wv in tables:
+        inc.append('static __constant__ const double %s[%d] = {%s}, %s[%d] = {%s};' % (xn, xl, xv, wn, wl, wv))
+    inc += ['#define RABS(x) sfx_abs(x)', code.strip('\n'), '#undef RABS', '']
     lines = ['// generated by sincfeyn_amd/emit/hipgen.py from %s -- do not edit' % os.path.basename(src_path),
              '// graph %s: %s, L=%d, D=%d, weight %d' % (name, desc['volkov'], desc['L'], desc['D'], desc['weight']),
              '#include "sfx_integrand.hpp"',
-             'namespace sfxk_%s {' % name]
-    for xn, xl, xv, wn, wl, wv in tables:
-        lines.append('static __constant__ const double %s[%d] = {%s}, %s[%d] = {%s};' % (xn, xl, xv, wn, wl, wv))
-    lines += ['#define RABS(x) sfx_abs(x)', code.strip('\n'), '#undef RABS',
-              'struct Graph {',
-              '    static constexpr int D = %d;' % desc['D'],
-              '    template<class REAL> static __device__ __forceinline__ REAL f(const REAL *sf, REAL *ab, const int sqab)',
-              '    { return f_%s<REAL>(sf, ab, sqab); }' % name,
-              '};',
-              '}  // namespace sfxk_%s' % name,
-              'SFX_GRAPH_ENTRY(%s, sfxk_%s::Graph, "%s", %d, %d, %d)' % (
-                  name, name, desc['volkov'].replace('"', '\\"'), desc['L'], desc['D'], desc['weight']),
-              '']
+             'namespace sfxk_%s {' % name,
+             '#include "k_%s.inc"' % name,
+             'struct Graph {',
+             '    static constexpr int D = %d;' % desc['D'],
+             '    template<class REAL> static __device__ __forceinline__ REAL f(const REAL *sf, REAL *ab, const int sqab)',
+             '    { return f_%s<REAL>(sf, ab, sqab); }' % name,
+             '};',
+             '}  // namespace sfxk_%s' % name,
+             'SFX_GRAPH_ENTRY(%s, sfxk_%s::Graph, "%s", %d, %d, %d)' % (
+                 name, name, desc['volkov'].replace('"', '\\"'), desc['L'], desc['D'], desc['weight']),
+             '']
     os.makedirs(out_dir, exist_ok=True)
+    for ext, body_lines in (('inc', inc), ('hip', lines)):
+        out = os.path.join(out_dir, 'k_%s.%s' % (name, ext))
+        text = '\n'.join(body_lines)
+        if os.path.exists(out) and open(out).read() == text:
+            continue                      # unchanged: keep the timestamp, so an incremental build skips it
+        with open(out + '.tmp', 'w') as f:
+            f.write(text)
+        os.replace(out + '.tmp', out)
     out = os.path.join(out_dir, 'k_%s.hip' % name)
-    tmp = out + '.tmp'
-    with open(tmp, 'w') as f:
-        f.write('\n'.join(lines))
-    os.replace(tmp, out)
     meta = dict(desc, source=os.path.basename(src_path), functions=len(names),
                 chunk_functions=sum(1 for n in names if _is_chunk(n)), static_ops=count_ops(body))
     with open(os.path.join(out_dir, 'k_%s.json' % name), 'w') as f:

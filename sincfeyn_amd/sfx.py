@@ -1,0 +1,212 @@
+"""sfx.py -- Python bindings of the C ABI (include/sfx.h) over ctypes.
+
+The library is sincfeyn_amd/_lib/libsfx.so, built by `python -m sincfeyn_amd.build` (or
+__graft_entry__.build()).  There is no fallback: if the library is missing or cannot initialise the GPU,
+every call raises.  The functions mirror the reference's interface for this path: hsum() is
+driver/sum.c's hsum() for one graph (with lattice_setup), run() is `sincqed run h N eps top GRAPH...`
+(driver/main.c:205-251) returning one record per graph with the fields of its results line
+(main.c:66-73), eval_nodes / eval_cells are the reference GPU back end's per-node primitive
+(driver/gpu.h:17-20).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, '_lib', 'libsfx.so')
+
+
+class SfxError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
+                ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
+                ('block_points', C.c_long), ('verbose', C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [('S', C.c_double * 3), ('Ah', C.c_double), ('ref', C.c_double), ('prune', C.c_double),
+                ('trunc', C.c_double), ('unres', C.c_double), ('t', C.c_double), ('nev', C.c_long), ('nq', C.c_long),
+                ('kept_top', C.c_long), ('es1', C.c_double), ('es2', C.c_double), ('esok', C.c_int), ('zero', C.c_int),
+                ('abskipped', C.c_long), ('unresdrop', C.c_long), ('btuns', C.c_long), ('nev_level', C.c_long * 16),
+                ('abserr', C.c_double), ('K', C.c_int), ('top', C.c_int), ('levels', C.c_int),
+                ('t_grow', C.c_double), ('t_dilate', C.c_double), ('t_refine', C.c_double)]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [('name', C.c_char * 32), ('volkov', C.c_char * 64), ('L', C.c_int), ('D', C.c_int), ('weight', C.c_int),
+                ('vgprs', C.c_int), ('sgprs', C.c_int), ('scratch_bytes', C.c_int), ('occupancy', C.c_int)]
+
+
+class ProfEntry(C.Structure):
+    _fields_ = [('graph', C.c_char * 32), ('kind', C.c_int), ('launches', C.c_long), ('ms', C.c_double),
+                ('points', C.c_long), ('skipped', C.c_long), ('dd_points', C.c_long)]
+
+
+_lib = None
+
+
+def lib():
+    """the loaded library (raises if it is not built)"""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SfxError('%s is not built: run python -m sincfeyn_amd.build' % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.sfx_init.argtypes = [C.c_int]
+    L.sfx_info.restype = C.c_char_p
+    L.sfx_last_error.restype = C.c_char_p
+    L.sfx_opts_default.argtypes = [C.POINTER(Opts)]
+    L.sfx_graph_info.argtypes = [C.c_int, C.POINTER(GraphInfo)]
+    L.sfx_graph_find.argtypes = [C.c_char_p]
+    L.sfx_graph_selected.argtypes = [C.c_int, C.c_char_p]
+    L.sfx_eval_nodes.argtypes = [C.c_int, C.c_double, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sfx_eval_cells.argtypes = [C.c_int, C.c_double, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.sfx_eval_raw.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_int, C.c_void_p]
+    L.sfx_eval_finished.argtypes = [C.c_int, C.c_double, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_double,
+                                    C.POINTER(Opts), C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
+    L.sfx_hsum.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(Opts), C.POINTER(Result)]
+    L.sfx_prof_enable.argtypes = [C.c_int]
+    L.sfx_prof_get.argtypes = [C.c_int, C.POINTER(ProfEntry)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise SfxError('sfx error %d: %s' % (rc, lib().sfx_last_error().decode()))
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        _check(lib().sfx_init(device))
+        _initialised = True
+    return lib().sfx_info().decode()
+
+
+def opts(**kw):
+    o = Opts()
+    lib().sfx_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def graphs():
+    """[{name, volkov, L, D, weight, ...}] in registry order"""
+    out = []
+    for i in range(lib().sfx_graph_count()):
+        g = GraphInfo()
+        _check(lib().sfx_graph_info(i, C.byref(g)))
+        out.append(dict(index=i, name=g.name.decode(), volkov=g.volkov.decode(), L=g.L, D=g.D, weight=g.weight,
+                        vgprs=g.vgprs, scratch_bytes=g.scratch_bytes, occupancy=g.occupancy))
+    return out
+
+
+def find(name):
+    i = lib().sfx_graph_find(name.encode())
+    if i < 0:
+        raise KeyError(name)
+    return i
+
+
+def select(*selectors):
+    """graphs.c:17-29 selection, in registry order without repeats (main.c:156-168)"""
+    L = lib()
+    sel = []
+    for s in selectors:
+        for i in range(L.sfx_graph_count()):
+            if L.sfx_graph_selected(i, s.encode()) and i not in sel:
+                sel.append(i)
+    return sel
+
+
+def default_top(h, topstep=3.0):
+    """main.c:211-215: 1, raised to 2 while the coarsest step 2^(top+1) h stays <= 3"""
+    top = 1
+    while top < 2 and (1 << (top + 1))*h <= topstep*(1 + 1e-9):
+        top += 1
+    return top
+
+
+def lattice_K(N, top):
+    """lattice.c:16"""
+    m = 1 << (top + 1)
+    K = (N - 1)//2
+    return ((K + m - 1)//m)*m
+
+
+def eval_raw(graph, sf, prec=0):
+    sf = np.ascontiguousarray(sf, dtype=np.float64)
+    n = sf.shape[0]
+    out = np.zeros((n, 4 if prec else 2))
+    _check(lib().sfx_eval_raw(graph, n, sf.ctypes.data, prec, out.ctypes.data))
+    return out
+
+
+def eval_nodes(graph, h, K, k, skip=None):
+    k = np.ascontiguousarray(k, dtype=np.int16)
+    n = k.shape[0]
+    fw = np.zeros(n); abw = np.zeros(n)
+    sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.uint8)
+    _check(lib().sfx_eval_nodes(graph, h, K, n, k.ctypes.data, None if sk is None else sk.ctypes.data,
+                                fw.ctypes.data, abw.ctypes.data))
+    return fw, abw
+
+
+def eval_cells(graph, h, K, cells, st, skip=None):
+    cells = np.ascontiguousarray(cells, dtype=np.int16)
+    nc, D = cells.shape
+    fw = np.zeros(nc << D); abw = np.zeros(nc << D)
+    sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.uint8)
+    _check(lib().sfx_eval_cells(graph, h, K, nc, cells.ctypes.data, None if sk is None else sk.ctypes.data, st,
+                                fw.ctypes.data, abw.ctypes.data))
+    return fw, abw
+
+
+def eval_finished(graph, h, K, k, abserr, skip=None, **kw):
+    k = np.ascontiguousarray(k, dtype=np.int16)
+    n = k.shape[0]
+    v = np.zeros(n); fl = np.zeros(n, dtype=np.uint8); nq = C.c_long()
+    sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.uint8)
+    o = opts(**kw)
+    _check(lib().sfx_eval_finished(graph, h, K, n, k.ctypes.data, None if sk is None else sk.ctypes.data, abserr,
+                                   C.byref(o), v.ctypes.data, fl.ctypes.data, C.byref(nq)))
+    return v, fl, nq.value
+
+
+def result_dict(r):
+    d = {k: getattr(r, k) for k, _ in r._fields_ if k not in ('S', 'nev_level')}
+    d['S'] = list(r.S)
+    d['nev_level'] = list(r.nev_level)
+    return d
+
+
+def hsum(graph, h, N, eps, top=None, **kw):
+    """hsum() of one graph on the device; graph = name or index"""
+    gi = find(graph) if isinstance(graph, str) else graph
+    top = default_top(h) if top is None else top
+    r = Result()
+    o = opts(**kw)
+    _check(lib().sfx_hsum(gi, h, N, eps, top, C.byref(o), C.byref(r)))
+    return result_dict(r)
+
+
+def graph_value(info, res):
+    """main.c:46-48: value = w S_h, err = w (disc + prune + trunc + round)"""
+    w = info['weight']
+    S0, S1 = res['S'][0], res['S'][1]
+    d2 = abs(S1 - S0) if not math.isnan(S1) else float('nan')
+    disc = 2*d2*d2/res['Ah'] if res['Ah'] > 0 else 0.0
+    return w*S0, w*(disc + res['prune'] + res['trunc'] + res['unres'])
