@@ -45,20 +45,40 @@ def test_all2_h08_top2(sfx, oracle, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('name', ['v3_00', 'v3_p4Sc', 'g3_07', 'g3_lbl'])
+@pytest.mark.parametrize('name', ['v3_00', 'v3_p4Sc', 'g3_07'])
 def test_three_loop_small(sfx, oracle, name):
     _cmp(sfx, oracle, name, 1.5, 33, 1e-12)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('name', ['v3_01', 'g3_07'])
-def test_three_loop_eps1e8(sfx, oracle, name):
-    """at eps = 1e-8 the reference accepts FP64 values whose rounding bound reaches ABSERR = 1e-3 eps Ref
-    per point (eval.c:36-38, sum.c:651), so two correct FP64 evaluation orders (the reference CPU build
-    contracts to FMA, the device does not) differ by more than 1e-12 in the sum; the counts must still
-    agree exactly, the value to 1e-9 (the reference's own GPU-vs-CPU agreement at eps = 1e-8 was
-    7.9e-9, docs/GPU.md:77-78)"""
-    _cmp(sfx, oracle, name, 1.5, 33, 1e-8, rel=1e-9)
+def test_three_loop_eps1e8(sfx, oracle, oracle_ref, name):
+    """eps = 1e-8.  Against the oracle without FP contraction: same counts, values to 1e-12 (every point is
+    bit-identical, only the summation order differs).  Against the reference's default build: the
+    reference accepts FP64 values whose rounding bound reaches ABSERR = 1e-3 eps Ref per point
+    (eval.c:36-38, sum.c:651), so two correct FP64 evaluation orders (FMA-contracted and not) differ by
+    more than 1e-12 in the sum; there the value must agree to 1e-9 (the reference's own GPU-vs-CPU
+    agreement at eps = 1e-8 was 7.9e-9, docs/GPU.md:77-78) and the counts still exactly on these graphs"""
+    _cmp(sfx, oracle, name, 1.5, 33, 1e-8)
+    _cmp(sfx, oracle_ref, name, 1.5, 33, 1e-8, rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_lbl_against_reference_nc():
+    """g3_lbl (25k FP64 operations per point): against the reference implementation itself built without
+    FP contraction (multi-threaded; the single-threaded oracle takes minutes here)"""
+    import oracle as O
+    from sincfeyn_amd import sfx as S
+    S.init(0)
+    h, N, eps = 2.0, 25, 1e-10
+    ref, wall, _ = O.run_reference(h, N, eps, ['g3_lbl'], top=1, env={'SINCQED_SYM': '0'}, nc=True)
+    R = ref['g3_lbl']
+    r = S.hsum('g3_lbl', h, N, eps, 1)
+    print('g3_lbl device %.17g ref %.17g nev %d/%d nq %d/%d (reference %.1fs, device %.2fs)' % (
+        r['S'][0], R['S0'], r['nev'], R['nev'], r['nq'], R['nq'], wall, r['t']), flush=True)
+    assert (r['nev'], r['nq'], r['kept_top']) == (R['nev'], R['nq'], R['kept'])
+    assert abs(r['S'][0] - R['S0']) <= REL*abs(R['S0'])
+    assert abs(r['S'][1] - R['S1']) <= REL*abs(R['S1'])
 
 
 @pytest.mark.gpu
