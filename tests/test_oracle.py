@@ -1,0 +1,128 @@
+"""CPU tests of the checkers: the reference build (oracle/_ref) against its committed golden outputs and
+the physics closed forms; the C restatement (oracle/sq_oracle.c) bit-exact against the reference, for
+both floating-point variants; the reference kernels against the committed per-point vectors."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, 'golden')
+ALL2 = ['g2_00', 'g2_01', 'g2_02', 'g2_03', 'v2_00']
+
+
+def _gold_records(fname):
+    recs, lines = {}, {}
+    for line in open(os.path.join(GOLD, fname)):
+        if line.startswith('#R '):
+            t = line.split()
+            recs[t[1]] = {k: float(v) for k, v in zip(t[2::2], t[3::2])}
+        elif line.strip() and not line.startswith('#'):
+            lines[line.split()[0]] = line.rstrip('\n')
+    return recs, lines
+
+
+needs_ref = pytest.mark.skipif(not os.path.exists(O.SINCQED_REF), reason='oracle/_ref not built (reference sources absent)')
+
+
+@needs_ref
+@pytest.mark.parametrize('mode,env', [('default', {}), ('sym0', {'SINCQED_SYM': '0'})])
+def test_reference_build_reproduces_golden_config1(mode, env):
+    """the reference built by oracle/Makefile gives the committed config-1 results (values to all printed
+    digits, every count; the time column is not compared)"""
+    recs, lines = _gold_records('config1_%s.txt' % mode)
+    out, wall, stdout = O.run_reference(1.0, 97, 1e-12, ['all2'], env=env, threads=4)
+    for g in ALL2:
+        for k in ('S0', 'S1', 'ref', 'Ah', 'prune', 'trunc', 'nev', 'nq', 'kept', 'skipped', 'unresdrop'):
+            assert out[g][k] == recs[g][k], (g, k, out[g][k], recs[g][k])
+
+
+def test_golden_config1_physics():
+    """config 1 against the exact two-loop values (reference graphs.c:36-46) within the reference's own
+    regression tolerance for this lattice (tests/run_tests.sh:20-22: the all2 sum at h = 1 within 3e-5 of
+    A2 = -0.328478965579, the Pi2 insertion within 3e-6) -- the physics, not the 1e-12 parity"""
+    recs, lines = _gold_records('config1_default.txt')
+    tot = 0.0
+    for g in ALL2:
+        tok = lines[g].split()
+        tot += float([t for t in tok if ('e-' in t or 'e+' in t) and len(t) > 18][0])
+    assert abs(tot - (-0.328478965579)) < 3e-5
+    v2 = float([t for t in lines['v2_00'].split() if ('e-' in t or 'e+' in t) and len(t) > 18][0])
+    assert abs(v2 - 0.015687421859) < 3e-6
+
+
+@pytest.fixture(scope='module')
+def orc():
+    if not O.available():
+        pytest.fail('oracle not built')
+    return {'ref': O.Oracle(nc=False), 'nc': O.Oracle(nc=True)}
+
+
+@needs_ref
+@pytest.mark.parametrize('variant', ['ref', 'nc'])
+def test_oracle_bitexact_vs_reference_config1(orc, variant):
+    """the restatement reproduces the reference implementation bit for bit (symmetry fold off)"""
+    ref, wall, _ = O.run_reference(1.0, 97, 1e-12, ['all2'], env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
+    for g in ALL2:
+        r = orc[variant].hsum(g, 1.0, 97, 1e-12, 1)
+        R = ref[g]
+        assert r['S'][0] == R['S0'] and r['S'][1] == R['S1'], g
+        assert (r['nev'], r['nq'], r['kept_top'], r['abskipped'], r['unresdrop']) == \
+               (R['nev'], R['nq'], R['kept'], R['skipped'], R['unresdrop']), g
+        for k in ('Ah', 'ref', 'prune', 'trunc'):
+            assert r[k] == R[k], (g, k)
+
+
+@needs_ref
+@pytest.mark.parametrize('variant', ['ref', 'nc'])
+@pytest.mark.parametrize('h,N,eps,graphs', [(1.5, 33, 1e-10, ['v3_00', 'v3_03', 'v3_p4Sc', 'v3_p4Vc']),
+                                            (0.7, 139, 1e-12, ['v2_00', 'g2_03'])])
+def test_oracle_bitexact_vs_reference_more(orc, variant, h, N, eps, graphs):
+    """three-loop VP graphs (incl. the value-branching Pi4 contact and the identically zero v3_p4Vc) and
+    top = 2 lattices"""
+    top = O.default_top(h)
+    ref, wall, _ = O.run_reference(h, N, eps, graphs, env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
+    for g in graphs:
+        r = orc[variant].hsum(g, h, N, eps, top)
+        R = ref[g]
+        assert (r['nev'], r['nq'], r['kept_top']) == (R['nev'], R['nq'], R['kept']), g
+        for j, k in ((0, 'S0'), (1, 'S1'), (2, 'S2')):
+            assert r['S'][j] == R[k] or (math.isnan(r['S'][j]) and math.isnan(R[k])), (g, k)
+
+
+def test_golden_points_reference_kernels(orc):
+    """the reference kernels as built here reproduce the committed per-point vectors exactly (so the
+    checkers' kernels are the ones the fixtures were made with)"""
+    gold = json.load(open(os.path.join(GOLD, 'points.json')))
+    nck = O.NcKernels()
+    for g, d in gold.items():
+        sf = np.array([float.fromhex(x) for x in d['sf']]).reshape(-1, d['D'])
+        f, ab = orc['ref'].eval_fd(g, sf)
+        assert [x.hex() for x in f] == d['fd'], g
+        f64, _ = nck.fp64(g, sf)
+        assert [x.hex() for x in f64] == d['fp64'], g
+        dd = nck.dd(g, sf)
+        assert [float(x).hex() for x in dd.ravel()] == d['dd'], g
+
+
+def test_dd_against_fp64_error_bound():
+    """the double-double values agree with the FP64 ones within the FP64 rounding bound the reference's
+    fallback test assumes (16 eps Sum|terms|, eval.c:36), and are far tighter: a dropped or wrong lo term
+    in the double-double arithmetic would show as a difference of FP64 size between dd and a better
+    evaluation; here the dd value must also agree with the FMA-contracted double half within that bound"""
+    gold = json.load(open(os.path.join(GOLD, 'points.json')))
+    for g, d in gold.items():
+        f = np.array([float.fromhex(x) for x in d['fd']])
+        f64 = np.array([float.fromhex(x) for x in d['fp64']])
+        ab = np.array([float.fromhex(x) for x in d['fp64_ab']])
+        dd = np.array([float.fromhex(x) for x in d['dd']]).reshape(-1, 4)
+        v = dd[:, 0] + dd[:, 1]
+        bound = 16*np.finfo(float).eps*ab
+        assert np.all(np.abs(v - f64) <= bound + 1e-300), g
+        assert np.all(np.abs(v - f) <= bound + 1e-300), g
+        # the pair is normalised: |lo| <= ulp(hi)/2
+        assert np.all(np.abs(dd[:, 1]) <= np.spacing(np.abs(dd[:, 0]))/2 + 1e-300), g
