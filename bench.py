@@ -8,6 +8,11 @@ plain per-graph path).  A point is one lattice-point evaluation as the reference
 
     python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--no-cpu-baseline]
 
+Two paths over the same jobs: the plain per-graph path (one hierarchical sum after another) and the
+batched path (all jobs submitted to the library's job engine, which runs them on a pool of workers with
+their own streams); `value` is the batched rate, `per_graph` the plain one, and every batched result is
+checked to equal its per-graph result.
+
 Prints one JSON line: points/s (value), per-job latency percentiles, the roofline of the dominant
 kernel (FP64 operations per point of each integrand x points per launch / mean launch duration, from
 HIP events on the library's stream), and the reference CPU implementation timed on a bounded sample of
@@ -42,13 +47,31 @@ def percentile(xs, p):
     return xs[f] + (xs[c] - xs[f])*(k - f)
 
 
-def run_pass(S, graphs, lat):
+def run_pass(S, graphs, lat, out=None):
+    """the plain per-graph path: one hierarchical sum after another on the library's stream"""
     ev = 0
     for gi, name in graphs:
         t0 = time.perf_counter()
         r = S.hsum(gi, H, N, EPS, TOP)
         lat.append(time.perf_counter() - t0)
         ev += r['nev']
+        if out is not None:
+            out[name] = r
+    return ev
+
+
+def run_batched(S, graphs, lat, out=None):
+    """the batched path: all jobs submitted at once to the engine (continuous batching over its workers);
+    a job's latency is submit -> result"""
+    ev = 0
+    t0 = time.perf_counter()
+    jobs = [(S.submit(gi, H, N, EPS, TOP), name) for gi, name in graphs]
+    for j, name in jobs:
+        r, tq, tr = S.wait(j)
+        lat.append(tq + tr)
+        ev += r['nev']
+        if out is not None:
+            out[name] = r
     return ev
 
 
@@ -111,6 +134,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--graphs', default='all3', help='selector (default all3); a subset is not the benchmark')
     ap.add_argument('--profile-out', default=None, help='write the per-kernel table as JSON here')
+    ap.add_argument('--workers', type=int, default=16, help='engine workers of the batched path')
+    ap.add_argument('--copy-probe', type=int, default=0,
+                    help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py: this build measures one GPU; the Sinc sum of one graph is not split across GPUs here '
@@ -120,32 +146,56 @@ def main():
     info = S.init(0)
     opc = json.load(open(os.path.join(ROOT, 'sincfeyn_amd', '_gen', 'opcounts.json')))
     graphs = [(i, S.graphs()[i]['name']) for i in S.select(*a.graphs.split(','))]
-    lat = []
+    if a.copy_probe:
+        ms = S.C.c_double()
+        S._check(S.lib().sfx_copy_probe(S.C.c_long(a.copy_probe), S.C.byref(ms)))
+        print('copy probe: %d bytes in %.3f ms' % (a.copy_probe, ms.value), file=sys.stderr)
+    # the plain per-graph path, profiled (kernel durations from HIP events on the library's stream)
+    lat_pg = []
     for _ in range(a.warmup):
         run_pass(S, graphs, [])
     S._check(S.lib().sfx_sync())
     S._check(S.lib().sfx_prof_enable(1))
     t0 = time.perf_counter()
-    ev = 0
+    ev_pg = 0
+    per_graph = {}
     for _ in range(a.steps):
-        ev += run_pass(S, graphs, lat)
+        ev_pg += run_pass(S, graphs, lat_pg, per_graph)
     S._check(S.lib().sfx_sync())
-    dt = time.perf_counter() - t0
+    dt_pg = time.perf_counter() - t0
     S._check(S.lib().sfx_prof_enable(0))
     rl, ent = roofline(S, opc)
     if a.profile_out:
         with open(a.profile_out, 'w') as f:
             json.dump(ent, f, indent=1)
+    # the batched path: the same jobs through the engine (headline value)
+    S.engine_start(a.workers, 0)
+    order = sorted(graphs, key=lambda g: -S.graphs()[g[0]]['D'])      # the widest lattices first
+    for _ in range(a.warmup):
+        run_batched(S, order, [])
+    lat = []
+    batched = {}
+    t0 = time.perf_counter()
+    ev = 0
+    for _ in range(a.steps):
+        ev += run_batched(S, order, lat, batched)
+    dt = time.perf_counter() - t0
+    same = all(batched[n]['S'] == per_graph[n]['S'] and batched[n]['nev'] == per_graph[n]['nev'] and
+               batched[n]['nq'] == per_graph[n]['nq'] for _, n in graphs)
     cpu = None if a.no_cpu_baseline else cpu_baseline()
     out = {
         'metric': 'Sinc-sum points/s batched vs per-graph; p50/p99 job latency at <=1e-12 rel',
         'value': ev/dt, 'unit': 'points/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
         'ms_per_step': dt*1e3/a.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f64', 'data': 'deterministic Sinc lattice of the named integrals (no input data)',
-        'config': {'workload': 'all41 three-loop integrals, per-graph path', 'graphs': len(graphs), 'h': H, 'N': N,
-                   'K': S.lattice_K(N, TOP), 'eps': EPS, 'top': TOP, 'points_per_step': ev//a.steps,
-                   'path': 'per-graph'},
+        'config': {'workload': 'all41 three-loop integrals (h=0.7, N=139, eps=1e-12), batched through the job engine',
+                   'graphs': len(graphs), 'h': H, 'N': N, 'K': S.lattice_K(N, TOP), 'eps': EPS, 'top': TOP,
+                   'points_per_step': ev//a.steps, 'path': 'batched', 'workers': a.workers},
         'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat)},
+        'per_graph': {'value': ev_pg/dt_pg, 'ms_per_step': dt_pg*1e3/a.steps,
+                      'latency_s': {'p50': percentile(lat_pg, 50), 'p99': percentile(lat_pg, 99)}},
+        'batched_over_per_graph': (ev/dt)/(ev_pg/dt_pg),
+        'batched_results_identical_to_per_graph': same,
         'roofline': rl,
         'cpu_baseline': cpu,
         'device': info,

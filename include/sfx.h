@@ -121,6 +121,26 @@ int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, cons
  * the device with the cell lists resident in HBM. */
 int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res);
 
+/* ---- job engine: continuous batching of hierarchical sums ----
+ * sfx_job_submit queues a job and returns at once; a pool of worker threads (sfx_engine_start, started
+ * with defaults by the first submit), each with its own HIP stream and device buffers, runs the queued
+ * jobs in arrival order, so jobs that arrive while others run are taken as workers free up and their
+ * device work overlaps.  Each refinement level reserves the HBM bound of its next cell list from the
+ * engine's budget (default 80% of the device memory) and waits while the budget is held by others.  A
+ * job's result does not depend on what runs beside it.  No counterpart in the reference, which runs
+ * graphs one after another in one process (main.c:243). */
+#define SFX_JOB_QUEUED  0
+#define SFX_JOB_RUNNING 1
+#define SFX_JOB_DONE    2
+#define SFX_JOB_FAILED  3
+int sfx_engine_start(int workers, double hbm_budget_bytes);      /* <= 0: 8 workers, 80% of HBM */
+int sfx_engine_stop(void);
+int sfx_job_submit(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, long *job);
+int sfx_job_poll(long job, int *state);
+/* blocks until the job has finished; *t_queue = seconds from submit to start, *t_run = start to end */
+int sfx_job_wait(long job, sfx_result_t *res, double *t_queue, double *t_run);
+int sfx_job_release(long job);
+
 /* ---- measurement ----
  * With profiling on, every device kernel the library launches is bracketed by HIP events on the
  * library's stream, and the elapsed times are accumulated per (graph, kernel kind) together with the
@@ -138,6 +158,9 @@ int sfx_prof_enable(int on);           /* on: clears the table and starts record
 int sfx_prof_count(void);
 int sfx_prof_get(int i, sfx_prof_entry_t *out);
 int sfx_sync(void);                    /* waits for the library's stream */
+/* a plain device-to-device copy of `bytes` bytes by the kernel k_copy_probe (reads `bytes`, writes
+ * `bytes`): the known-size reference that calibrates hardware traffic counters (FETCH_SIZE, WRITE_SIZE) */
+int sfx_copy_probe(long bytes, double *ms);
 
 #ifdef __cplusplus
 }

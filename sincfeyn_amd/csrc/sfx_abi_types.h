@@ -30,6 +30,7 @@ typedef struct {
 #define SFX_FL_NEAR  2u      /* "near" the fallback (eval.c:37) */
 #define SFX_FL_DD    4u      /* needs the double-double recompute */
 #define SFX_FL_SKIP  8u      /* evaluated without the |.| sums */
+#define SFX_FL_UNRES 16u     /* unresolved even in double-double (set by the fallback kernel) */
 
 #ifdef __cplusplus
 }

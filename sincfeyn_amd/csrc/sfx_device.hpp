@@ -39,7 +39,7 @@ struct DevBuf {
 struct ddacc { double hi, lo; };
 
 // per-tile partial of a refinement block (see k_level_partials)
-struct LevelPart { double s[2], a[2], b1[2], b2[2], drop[2]; uint32_t nv, nskip, nkeep, pad; };
+struct LevelPart { double s[2], a[2], b1[2], b2[2], drop[2], u[2]; uint32_t nv, nskip, nkeep, nq, nd, pad; };
 // level accumulator in device memory, added to by every block of the level in block order
 struct LevelAcc {
     double s[2], a[2], b1[2], b2[2], drop[2], u[2];
@@ -60,10 +60,11 @@ hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask,
 // skip bit, the child's near flag and tier, sum.c:744) and the others add |v| pw to the dropped weight.
 // The double-double fallback results (e2, nd of the ddcnt fallback points) are reduced too.  Everything
 // is added to *acc in a fixed order; nxt receives the kept cells at offset acc->nkept (before the call).
+// e2[i] is read for the children whose flag has SFX_FL_DD (written there by the fallback kernel).
 struct LevelArgs {
     const double *v; const uint8_t *fl; const sfx_cell_t *cells; long n; int D; int st; int last; int abskip;
     double pw, thr;
-    const double *e2; const uint8_t *nd; const uint32_t *ddcnt;
+    const double *e2;
     sfx_cell_t *nxt; long nxt_cap;
     LevelPart *part; uint32_t *keepoff;             // scratch: ceil(n/TILE) entries each
     LevelAcc *acc;

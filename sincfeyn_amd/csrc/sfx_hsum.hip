@@ -15,6 +15,12 @@
 #include <algorithm>
 #include <chrono>
 #include <vector>
+#include <mutex>
+#include <thread>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <atomic>
 #include "sfx_device.hpp"
 
 namespace sfx {
@@ -26,22 +32,27 @@ struct JobBufs {
     void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc};
                     for(DevBuf *b : all) b->release(); }
 };
-static JobBufs &bufs(){ static JobBufs B; return B; }
+// a context: one HIP stream and its device buffers; the library's own context serves sfx_hsum and the
+// per-point calls, each engine worker has its own (so jobs on different workers overlap on the device)
+struct Ctx {
+    hipStream_t s = nullptr; JobBufs B; hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+static Ctx &default_ctx(){ static Ctx C; C.s = lib().stream; return C; }
 
 // ---- profiling: HIP events on the library stream around each kernel, accumulated per (graph, kind)
 struct Prof {
     bool on = false;
     std::vector<sfx_prof_entry_t> table;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::mutex mu;
 };
 static Prof &prof(){ static Prof P; return P; }
-static int prof_events(){
-    Prof &P = prof();
-    for(int i = 0; i < 4; i++) if(!P.ev[i]) SFX_HIP(hipEventCreate(&P.ev[i]));
+static int prof_events(Ctx &c){
+    for(int i = 0; i < 4; i++) if(!c.ev[i]) SFX_HIP(hipEventCreate(&c.ev[i]));
     return SFX_OK;
 }
 static void prof_add(const char *graph, int kind, double ms, long points, long skipped, long dd){
     Prof &P = prof();
+    std::lock_guard<std::mutex> lk(P.mu);
     for(sfx_prof_entry_t &e : P.table) if(e.kind == kind && !strcmp(e.graph, graph)){
         e.launches++; e.ms += ms; e.points += points; e.skipped += skipped; e.dd_points += dd; return;
     }
@@ -50,7 +61,7 @@ static void prof_add(const char *graph, int kind, double ms, long points, long s
     e.launches = 1; e.ms = ms; e.points = points; e.skipped = skipped; e.dd_points = dd;
     P.table.push_back(e);
 }
-static double prof_ms(int a, int b){ float ms = 0; (void)hipEventElapsedTime(&ms, prof().ev[a], prof().ev[b]); return ms; }
+static double prof_ms(Ctx &c, int a, int b){ float ms = 0; (void)hipEventElapsedTime(&ms, c.ev[a], c.ev[b]); return ms; }
 
 // Neumaier compensated sum (reference sum.c:18-20)
 struct Neu {
@@ -91,7 +102,7 @@ struct KeyMap {
 struct Job {
     const sfx_graph_entry *g; int gi, D, K, top; double h, eps;
     sfx_opts_t o; sfx_eval_params_t P;
-    hipStream_t s;
+    hipStream_t s; Ctx *ctx;
 };
 
 // ---------------------------------------------------------------------------------------------------
@@ -100,7 +111,7 @@ struct Job {
 // unresolved count, accumulated in list order
 static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v, uint8_t *flg, long *nq, double *u, long *nd){
     if(n <= 0) return SFX_OK;
-    JobBufs &B = bufs(); int rc;
+    JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; int rc;
     long nb = (n + TILE - 1)/TILE;
     if((rc = B.pts.reserve(n*sizeof(sfx_cell_t))) || (rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) ||
        (rc = B.ddblk.reserve((2*nb + 2)*sizeof(uint32_t))) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
@@ -108,29 +119,29 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
     hipStream_t s = J.s;
     SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
     const bool pon = prof().on;
-    if(pon){ if((rc = prof_events())) return rc; SFX_HIP(hipEventRecord(prof().ev[0], s)); }
+    if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
     SFX_HIP(J.g->eval(s, 0, B.pts.as<sfx_cell_t>(), n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
-    if(pon) SFX_HIP(hipEventRecord(prof().ev[1], s));
+    if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     SFX_HIP(J.g->dd(s, 0, B.pts.as<sfx_cell_t>(), 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
-                    B.v.as<double>(), B.e2.as<double>(), B.nd.as<uint8_t>()));
-    if(pon) SFX_HIP(hipEventRecord(prof().ev[2], s));
+                    B.v.as<double>(), B.e2.as<double>(), B.fl.as<uint8_t>()));
+    if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
     uint32_t m = 0;
     std::vector<uint8_t> fl(n);
     SFX_HIP(hipMemcpyAsync(v, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
     SFX_HIP(hipMemcpyAsync(fl.data(), B.fl.p, n, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipMemcpyAsync(&m, B.ddcnt.p, sizeof m, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
-    std::vector<double> e2(m); std::vector<uint8_t> ndv(m); std::vector<uint32_t> idx(m);
+    std::vector<double> e2; std::vector<uint32_t> idx(m);
     if(m){
-        SFX_HIP(hipMemcpyAsync(e2.data(), B.e2.p, m*sizeof(double), hipMemcpyDeviceToHost, s));
-        SFX_HIP(hipMemcpyAsync(ndv.data(), B.nd.p, m, hipMemcpyDeviceToHost, s));
+        e2.resize(n);
+        SFX_HIP(hipMemcpyAsync(e2.data(), B.e2.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
         SFX_HIP(hipMemcpyAsync(idx.data(), B.ddidx.p, m*sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         SFX_HIP(hipStreamSynchronize(s));
     }
-    if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(1, 2), m, 0, m); }
+    if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m); }
     for(long i = 0; i < n; i++) flg[i] = fl[i] & (SFX_FL_NEAR | SFX_FL_DD);
-    for(uint32_t j = 0; j < m; j++){ *u += e2[j]; if(ndv[j]){ (*nd)++; flg[idx[j]] |= SFX_PT_UNRES; } }
+    for(uint32_t j = 0; j < m; j++){ long i = idx[j]; *u += e2[i]; if(fl[i] & SFX_FL_UNRES){ (*nd)++; flg[i] |= SFX_PT_UNRES; } }
     *nq += m;
     return SFX_OK;
 }
@@ -314,7 +325,7 @@ static int box_zero(Job &J, int lev, int *zero, long *nev, long *unresdrop){
 
 static void set_params(Job &J, double abserr){
     J.P.K = J.K; J.P.H = J.h; J.P.abskip = J.o.abskip; J.P.abserr = abserr; J.P.nearf = J.o.nearf;
-    J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.EX = bufs().EX.as<double>();
+    J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.EX = J.ctx->B.EX.as<double>();
 }
 
 // grow a device buffer keeping its first `used` bytes (DevBuf::reserve does not preserve contents)
@@ -332,7 +343,7 @@ __global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0)
 
 // one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
 static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
-    JobBufs &B = bufs(); hipStream_t s = J.s; int rc;
+    JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; hipStream_t s = J.s; int rc;
     const int D = J.D, st = 1 << lev; const bool last = lev == 0;
     const double pw = pow((double)st, (double)D), thr = J.eps*ref*1e-2;
     long maxpts = J.o.block_points > 0 ? J.o.block_points : (1L << 24);
@@ -348,30 +359,29 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
         const bool pon = prof().on;
-        if(pon){ if((rc = prof_events())) return rc; SFX_HIP(hipEventRecord(prof().ev[0], s)); }
+        if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
         SFX_HIP(J.g->eval(s, 1, cells, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
-        if(pon) SFX_HIP(hipEventRecord(prof().ev[1], s));
+        if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
         SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
         SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P, B.v.as<double>(),
-                        B.e2.as<double>(), B.nd.as<uint8_t>()));
-        if(pon) SFX_HIP(hipEventRecord(prof().ev[2], s));
+                        B.e2.as<double>(), B.fl.as<uint8_t>()));
+        if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
         // room for every child of this block in the next level's list (kept so far is known exactly)
         if(!last && (rc = reserve_preserving(B.nxt, (size_t)kept*sizeof(sfx_cell_t), (size_t)(kept + n)*sizeof(sfx_cell_t), s))) return rc;
         LevelArgs a;
         a.v = B.v.as<double>(); a.fl = B.fl.as<uint8_t>(); a.cells = cells; a.n = n; a.D = D; a.st = st; a.last = last;
-        a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nd = B.nd.as<uint8_t>();
-        a.ddcnt = B.ddcnt.as<uint32_t>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
+        a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
         a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<uint32_t>(); a.acc = B.acc.as<LevelAcc>();
         SFX_HIP(level_reduce(s, a));
         if(pon){
-            SFX_HIP(hipEventRecord(prof().ev[3], s));
+            SFX_HIP(hipEventRecord(X.ev[3], s));
             LevelAcc now_; SFX_HIP(hipMemcpyAsync(&now_, B.acc.p, sizeof now_, hipMemcpyDeviceToHost, s));
             SFX_HIP(hipStreamSynchronize(s));
             long dv = (long)now_.nv - prof_nv, dsk = (long)now_.nskip - prof_nsk, dq = (long)now_.nq - prof_nq;
             prof_nv = (long)now_.nv; prof_nsk = (long)now_.nskip; prof_nq = (long)now_.nq;
-            prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(0, 1), dv, dsk, 0);
-            prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(1, 2), dq, 0, dq);
-            prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(2, 3), n, 0, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(X, 0, 1), dv, dsk, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(X, 1, 2), dq, 0, dq);
+            prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(X, 2, 3), n, 0, 0);
         }
         if(!last){
             unsigned long long k = 0;
@@ -391,7 +401,8 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
 
 using namespace sfx;
 
-extern "C" void sfx_release_job_buffers(void){ bufs().release(); }
+namespace sfx { void engine_shutdown(); }
+extern "C" void sfx_release_job_buffers(void){ engine_shutdown(); default_ctx().B.release(); }
 
 extern "C" int sfx_prof_enable(int on){
     Prof &P = prof();
@@ -417,7 +428,8 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     if(graph < 0 || graph >= SFX_NGRAPHS || n < 0 || K < 1 || K > 127 || !(h > 0) || (n && (!k || !v || !flags))){
         set_error("sfx_eval_finished: bad arguments"); return SFX_EINVAL;
     }
-    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.K = K; J.eps = 0; J.top = 0; J.s = lib().stream;
+    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.K = K; J.eps = 0; J.top = 0;
+    J.ctx = &default_ctx(); J.s = J.ctx->s;
     if(opts) J.o = *opts; else sfx_opts_default(&J.o);
     std::vector<sfx_cell_t> pts(n);
     for(long i = 0; i < n; i++){
@@ -431,8 +443,8 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     }
     std::vector<double> EX(2*K + 1);
     for(int x = -K; x <= K; x++) EX[x + K] = exp(x*h);
-    if((rc = bufs().EX.reserve(EX.size()*sizeof(double)))) return rc;
-    SFX_HIP(hipMemcpyAsync(bufs().EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
+    if((rc = J.ctx->B.EX.reserve(EX.size()*sizeof(double)))) return rc;
+    SFX_HIP(hipMemcpyAsync(J.ctx->B.EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
     set_params(J, abserr);
     long q = 0, nd = 0; double u = 0;
     if((rc = eval_nodes_finished(J, pts.data(), n, v, flags, &q, &u, &nd))) return rc;
@@ -440,18 +452,25 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     return SFX_OK;
 }
 
-extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
+static int check_job(int graph, double h, int N, double eps, int top){
+    if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("bad graph index %d", graph); return SFX_EINVAL; }
+    if(top < 1 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("need h > 0, N >= 3, eps > 0, 1 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
+    int m = 1 << (top + 1), K = (N - 1)/2; K = ((K + m - 1)/m)*m;
+    if(K > 127){ set_error("N = %d gives K = %d; points are stored in 8 bits, so K <= 127 (N <= 255)", N, K); return SFX_EINVAL; }
+    return SFX_OK;
+}
+
+namespace sfx { void budget_acquire(double bytes); void budget_release(double bytes); }
+
+// hsum of one graph on the context's stream (arguments checked by the caller)
+static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
     int rc;
-    if((rc = require_init())) return rc;
-    if(graph < 0 || graph >= SFX_NGRAPHS || !res){ set_error("sfx_hsum: bad graph or result pointer"); return SFX_EINVAL; }
-    if(top < 1 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("sfx_hsum: need h > 0, N >= 3, eps > 0, 1 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
-    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.eps = eps; J.top = top; J.s = lib().stream;
+    Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.eps = eps; J.top = top; J.ctx = &X; J.s = X.s;
     if(opts) J.o = *opts; else sfx_opts_default(&J.o);
     int m = 1 << (top + 1); J.K = (N - 1)/2; J.K = ((J.K + m - 1)/m)*m;      // lattice.c:16
-    if(J.K > 127){ set_error("sfx_hsum: N = %d gives K = %d; points are stored in 8 bits, so K <= 127 (N <= 255)", N, J.K); return SFX_EINVAL; }
     memset(res, 0, sizeof *res);
     res->K = J.K; res->top = top; res->S[1] = res->S[2] = NAN;
-    JobBufs &B = bufs();
+    JobBufs &B = X.B;
     double t0 = now();
     // the EX table on the host with exp() (lattice.c:18-19), uploaded as is
     std::vector<double> EX(2*J.K + 1);
@@ -483,9 +502,14 @@ extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const s
     res->abserr = abserr;
     set_params(J, abserr);
     res->esok = J.D < 12;
+    double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
+    budget_acquire(held);
     for(int lev = top - 1; lev >= 0; lev--){
         long nnext = 0; LevelAcc A;
-        if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))) return rc;
+        double want = lev > 0 ? (double)ncur*(double)(1L << J.D)*sizeof(sfx_cell_t) : 0;   // bound of the next list
+        budget_acquire(want);
+        if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))){ budget_release(held + want); return rc; }
+        if(lev > 0){ double now_ = (double)nnext*sizeof(sfx_cell_t); budget_release(held + want - now_); held = now_; }
         double S = A.s[0] + A.s[1];
         res->prune += A.drop[0] + A.drop[1];
         res->nev += (long)A.nv; res->nq += (long)A.nq; res->unresdrop += (long)A.nd;
@@ -498,9 +522,181 @@ extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const s
         }
         if(lev > 0){ std::swap(B.cur, B.nxt); ncur = nnext; }
     }
+    budget_release(held);
     res->levels = nl;
     res->S[0] = Slev[nl - 1]; res->S[1] = nl > 1 ? Slev[nl - 2] : NAN; res->S[2] = nl > 2 ? Slev[nl - 3] : NAN;
     res->t_refine = now() - t2;
     res->t = now() - t0;
+    return SFX_OK;
+}
+
+extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
+    int rc;
+    if((rc = require_init())) return rc;
+    if(!res){ set_error("sfx_hsum: null result"); return SFX_EINVAL; }
+    if((rc = check_job(graph, h, N, eps, top))) return rc;
+    return hsum_core(default_ctx(), graph, h, N, eps, top, opts, res);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// job engine: a pool of workers, each with its own HIP stream and device buffers, taking (graph, h, N,
+// eps, top) jobs from a FIFO queue.  A job's coarsest level runs on its worker's host thread (device
+// evaluations on the worker's stream) while other workers' refinements run on the device, so host and
+// device work of different jobs overlap and kernels of different jobs run concurrently.  Admission of a
+// refinement level is by its HBM footprint: before a non-last level, the job reserves the bound of its
+// next cell list (every child kept: n_cells 2^D x 16 B) from the engine's budget and waits while other
+// jobs hold it; the reservation shrinks to the actual list after the level.
+namespace sfx {
+
+struct Budget {
+    std::mutex mu; std::condition_variable cv; double cap = 0, used = 0;
+};
+static Budget &budget(){ static Budget B; return B; }
+void budget_acquire(double bytes){
+    Budget &B = budget(); if(B.cap <= 0 || bytes <= 0) return;
+    std::unique_lock<std::mutex> lk(B.mu);
+    B.cv.wait(lk, [&]{ return B.used <= 0 || B.used + bytes <= B.cap; });
+    B.used += bytes;
+}
+void budget_release(double bytes){
+    Budget &B = budget(); if(B.cap <= 0 || bytes <= 0) return;
+    { std::lock_guard<std::mutex> lk(B.mu); B.used -= bytes; if(B.used < 0) B.used = 0; }
+    B.cv.notify_all();
+}
+
+struct EJob {
+    int graph = 0, N = 0, top = 0; double h = 0, eps = 0; sfx_opts_t o;
+    int state = SFX_JOB_QUEUED, rc = 0; char err[256] = "";
+    sfx_result_t res;
+    double t_submit = 0, t_start = 0, t_end = 0;
+};
+struct Engine {
+    std::mutex mu; std::condition_variable cv, done;
+    std::deque<long> queue;
+    std::map<long, EJob> jobs;
+    std::vector<std::thread> workers;
+    std::vector<Ctx*> ctx;
+    bool stop = false; long next = 1; int running = 0;
+    // at process exit the idle workers are woken and joined (a joinable std::thread must not be destroyed);
+    // the HIP objects are left to the runtime's own teardown
+    ~Engine(){
+        { std::lock_guard<std::mutex> lk(mu); stop = true; }
+        cv.notify_all();
+        for(std::thread &t : workers) if(t.joinable()) t.join();
+    }
+};
+static Engine &eng(){ static Engine E; return E; }
+
+static void worker_main(int w){
+    Engine &E = eng();
+    (void)hipSetDevice(lib().device);
+    Ctx &X = *E.ctx[w];
+    for(;;){
+        long id; EJob *j;
+        {
+            std::unique_lock<std::mutex> lk(E.mu);
+            E.cv.wait(lk, [&]{ return E.stop || !E.queue.empty(); });
+            if(E.stop) return;
+            id = E.queue.front(); E.queue.pop_front();
+            j = &E.jobs[id]; j->state = SFX_JOB_RUNNING; j->t_start = now(); E.running++;
+        }
+        sfx_result_t r;
+        int rc = hsum_core(X, j->graph, j->h, j->N, j->eps, j->top, &j->o, &r);
+        {
+            std::lock_guard<std::mutex> lk(E.mu);
+            j->res = r; j->rc = rc; j->t_end = now();
+            if(rc) snprintf(j->err, sizeof j->err, "%s", sfx_last_error());
+            j->state = rc ? SFX_JOB_FAILED : SFX_JOB_DONE; E.running--;
+        }
+        E.done.notify_all();
+    }
+}
+
+static int engine_start(int workers, double budget_bytes){
+    Engine &E = eng();
+    std::lock_guard<std::mutex> lk(E.mu);
+    if(!E.workers.empty()) return SFX_OK;
+    if(workers <= 0) workers = 8;
+    if(workers > 64) workers = 64;
+    budget().cap = budget_bytes > 0 ? budget_bytes : 0.8*(double)lib().prop.totalGlobalMem;
+    E.stop = false;
+    for(int w = 0; w < workers; w++){
+        Ctx *c = new Ctx();
+        hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+        if(e != hipSuccess){ delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
+        E.ctx.push_back(c);
+    }
+    for(int w = 0; w < workers; w++) E.workers.emplace_back(worker_main, w);
+    return SFX_OK;
+}
+
+void engine_shutdown(){
+    Engine &E = eng();
+    { std::lock_guard<std::mutex> lk(E.mu); E.stop = true; }
+    E.cv.notify_all();
+    for(std::thread &t : E.workers) t.join();
+    E.workers.clear();
+    for(Ctx *c : E.ctx){ (void)hipStreamSynchronize(c->s); c->B.release(); for(hipEvent_t &v : c->ev) if(v) (void)hipEventDestroy(v);
+                         (void)hipStreamDestroy(c->s); delete c; }
+    E.ctx.clear();
+    std::lock_guard<std::mutex> lk(E.mu);
+    E.queue.clear(); E.jobs.clear(); E.running = 0;
+}
+
+}  // namespace sfx
+
+extern "C" int sfx_engine_start(int workers, double hbm_budget_bytes){
+    int rc; if((rc = require_init())) return rc;
+    return engine_start(workers, hbm_budget_bytes);
+}
+extern "C" int sfx_engine_stop(void){ engine_shutdown(); return SFX_OK; }
+
+extern "C" int sfx_job_submit(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, long *job){
+    int rc;
+    if((rc = require_init())) return rc;
+    if(!job){ set_error("sfx_job_submit: null job pointer"); return SFX_EINVAL; }
+    if((rc = check_job(graph, h, N, eps, top))) return rc;
+    if(eng().workers.empty() && (rc = engine_start(0, 0))) return rc;
+    Engine &E = eng();
+    {
+        std::lock_guard<std::mutex> lk(E.mu);
+        long id = E.next++;
+        EJob &j = E.jobs[id];
+        j.graph = graph; j.h = h; j.N = N; j.eps = eps; j.top = top; j.t_submit = now();
+        if(opts) j.o = *opts; else sfx_opts_default(&j.o);
+        E.queue.push_back(id);
+        *job = id;
+    }
+    E.cv.notify_one();
+    return SFX_OK;
+}
+
+extern "C" int sfx_job_poll(long job, int *state){
+    Engine &E = eng(); std::lock_guard<std::mutex> lk(E.mu);
+    auto it = E.jobs.find(job);
+    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
+    if(state) *state = it->second.state;
+    return SFX_OK;
+}
+
+extern "C" int sfx_job_wait(long job, sfx_result_t *res, double *t_queue, double *t_run){
+    Engine &E = eng(); std::unique_lock<std::mutex> lk(E.mu);
+    auto it = E.jobs.find(job);
+    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
+    EJob &j = it->second;
+    E.done.wait(lk, [&]{ return j.state == SFX_JOB_DONE || j.state == SFX_JOB_FAILED; });
+    if(res) *res = j.res;
+    if(t_queue) *t_queue = j.t_start - j.t_submit;
+    if(t_run) *t_run = j.t_end - j.t_start;
+    if(j.rc){ set_error("%s", j.err); return j.rc; }
+    return SFX_OK;
+}
+
+extern "C" int sfx_job_release(long job){
+    Engine &E = eng(); std::lock_guard<std::mutex> lk(E.mu);
+    auto it = E.jobs.find(job);
+    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
+    if(it->second.state == SFX_JOB_QUEUED || it->second.state == SFX_JOB_RUNNING){ set_error("job %ld not finished", job); return SFX_EINVAL; }
+    E.jobs.erase(it);
     return SFX_OK;
 }

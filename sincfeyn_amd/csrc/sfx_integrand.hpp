@@ -21,7 +21,9 @@
 #include "sfx_abi_types.h"
 
 #define SFX_FN __device__ __forceinline__
-#define SFX_CHUNK __device__ __noinline__
+#ifndef SFX_CHUNK
+#define SFX_CHUNK __device__ __noinline__     // the forest-block functions of the largest integrands
+#endif
 
 #ifndef SFX_BLOCK
 #define SFX_BLOCK 256
@@ -88,11 +90,12 @@ k_eval(const sfx_cell_t *__restrict__ pts, long n, int st, sfx_eval_params_t P, 
 }
 
 // the double-double recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
-// the count); writes the value, the eps^2 contribution to the round column and the "unresolved" mark
+// the count); writes, at the point's own index i, the value, the eps^2 contribution to the round column
+// and the "unresolved" flag bit
 template<class G, bool CELLS>
 __global__ void __launch_bounds__(SFX_BLOCK)
 k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ cnt,
-     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ ndout){
+     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ flout){
     constexpr int D = G::D;
     long m = *cnt;
     for(long j = blockIdx.x*(long)blockDim.x + threadIdx.x; j < m; j += (long)gridDim.x*blockDim.x){
@@ -109,9 +112,8 @@ k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ id
         double e = abqd*w*SFX_QEPS;                                   // eval.c:26
         double e2 = (e*16 <= P.abserr) ? 0.0 : (isfinite(e) ? e*e : 0.0);
         double v = isfinite(vqd) ? vqd*w : 0;
-        uint8_t nd = 0;
-        if(!(e <= fabs(v))){ nd = 1; if(P.dropunres) v = 0; }
-        vout[i] = v; e2out[j] = e2; ndout[j] = nd;
+        if(!(e <= fabs(v))){ flout[i] |= (uint8_t)SFX_FL_UNRES; if(P.dropunres) v = 0; }
+        vout[i] = v; e2out[i] = e2;
     }
 }
 

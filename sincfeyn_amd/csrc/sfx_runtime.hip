@@ -120,8 +120,8 @@ __device__ __forceinline__ int tier(double f, double t){ return (f >= 10*t) + (f
 
 __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
     long base = (long)blockIdx.x*TILE;
-    double s[2] = {0, 0}, ab[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0}, dr[2] = {0, 0};
-    unsigned nv = 0, nsk = 0, nk = 0;
+    double s[2] = {0, 0}, ab[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0}, dr[2] = {0, 0}, u[2] = {0, 0};
+    unsigned nv = 0, nsk = 0, nk = 0, nq = 0, nd = 0;
     for(int j = 0; j < TILE_ITEMS; j++){
         long i = base + j*TILE_THREADS + threadIdx.x;
         if(i >= a.n) break;
@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
         double v = a.v[i], x = v*a.pw, av = fabs(v);
         unsigned cfl = a.cells[i >> a.D].fl;
         nv++; nsk += (a.abskip && (cfl & 1u)) ? 1u : 0u;
+        if(f & SFX_FL_DD){ nq++; nd += (f & SFX_FL_UNRES) ? 1u : 0u; dacc_add(u[0], u[1], a.e2[i]); }
         dacc_add(s[0], s[1], x); dacc_add(ab[0], ab[1], av);
         if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(b1[0], b1[1], x); if(te >= 2) dacc_add(b2[0], b2[1], x); }
         else if(av >= a.thr) nk++;
@@ -142,13 +143,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
         dacc_addd(b1[0], b1[1], shfl_down(b1[0], d), shfl_down(b1[1], d));
         dacc_addd(b2[0], b2[1], shfl_down(b2[0], d), shfl_down(b2[1], d));
         dacc_addd(dr[0], dr[1], shfl_down(dr[0], d), shfl_down(dr[1], d));
+        dacc_addd(u[0], u[1], shfl_down(u[0], d), shfl_down(u[1], d));
         nv += __shfl_down(nv, d, 64); nsk += __shfl_down(nsk, d, 64); nk += __shfl_down(nk, d, 64);
+        nq += __shfl_down(nq, d, 64); nd += __shfl_down(nd, d, 64);
     }
     __shared__ LevelPart sh[TILE_THREADS/64];
     if(__lane_id() == 0){
         LevelPart &p = sh[threadIdx.x >> 6];
         p.s[0] = s[0]; p.s[1] = s[1]; p.a[0] = ab[0]; p.a[1] = ab[1]; p.b1[0] = b1[0]; p.b1[1] = b1[1];
-        p.b2[0] = b2[0]; p.b2[1] = b2[1]; p.drop[0] = dr[0]; p.drop[1] = dr[1]; p.nv = nv; p.nskip = nsk; p.nkeep = nk;
+        p.b2[0] = b2[0]; p.b2[1] = b2[1]; p.drop[0] = dr[0]; p.drop[1] = dr[1]; p.u[0] = u[0]; p.u[1] = u[1];
+        p.nv = nv; p.nskip = nsk; p.nkeep = nk; p.nq = nq; p.nd = nd;
     }
     __syncthreads();
     if(threadIdx.x == 0){
@@ -157,33 +161,30 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
             const LevelPart &q = sh[w];
             dacc_addd(t.s[0], t.s[1], q.s[0], q.s[1]); dacc_addd(t.a[0], t.a[1], q.a[0], q.a[1]);
             dacc_addd(t.b1[0], t.b1[1], q.b1[0], q.b1[1]); dacc_addd(t.b2[0], t.b2[1], q.b2[0], q.b2[1]);
-            dacc_addd(t.drop[0], t.drop[1], q.drop[0], q.drop[1]);
-            t.nv += q.nv; t.nskip += q.nskip; t.nkeep += q.nkeep;
+            dacc_addd(t.drop[0], t.drop[1], q.drop[0], q.drop[1]); dacc_addd(t.u[0], t.u[1], q.u[0], q.u[1]);
+            t.nv += q.nv; t.nskip += q.nskip; t.nkeep += q.nkeep; t.nq += q.nq; t.nd += q.nd;
         }
         a.part[blockIdx.x] = t;
         a.keepoff[blockIdx.x] = t.nkeep;
     }
 }
 
-// one block of CB threads: the tiles' partials in tile order, the fallback points' e^2 and unresolved
-// marks in list order, the kept counts scanned into offsets; everything added to the level accumulator
+// one block of CB threads: the tiles' partials in tile order, the kept counts scanned into offsets;
+// everything added to the level accumulator
 constexpr int CB = 256;
 __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
     __shared__ double sh[6][2][CB];
-    __shared__ unsigned long long shc[4][CB];
+    __shared__ unsigned long long shc[5][CB];
     long per = (nb + CB - 1)/CB, b0 = threadIdx.x*per, b1 = b0 + per < nb ? b0 + per : nb;
     double v[6][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    unsigned long long nv = 0, nsk = 0, nk = 0, nd = 0;
+    unsigned long long nv = 0, nsk = 0, nk = 0, nd = 0, nq = 0;
     for(long b = b0; b < b1; b++){
         const LevelPart &p = a.part[b];
         dacc_addd(v[0][0], v[0][1], p.s[0], p.s[1]); dacc_addd(v[1][0], v[1][1], p.a[0], p.a[1]);
         dacc_addd(v[2][0], v[2][1], p.b1[0], p.b1[1]); dacc_addd(v[3][0], v[3][1], p.b2[0], p.b2[1]);
-        dacc_addd(v[4][0], v[4][1], p.drop[0], p.drop[1]);
-        nv += p.nv; nsk += p.nskip; nk += p.nkeep;
+        dacc_addd(v[4][0], v[4][1], p.drop[0], p.drop[1]); dacc_addd(v[5][0], v[5][1], p.u[0], p.u[1]);
+        nv += p.nv; nsk += p.nskip; nk += p.nkeep; nq += p.nq; nd += p.nd;
     }
-    // fallback points, in list order
-    long m = a.ddcnt ? (long)*a.ddcnt : 0, perq = (m + CB - 1)/CB, q0 = threadIdx.x*perq, q1 = q0 + perq < m ? q0 + perq : m;
-    for(long j = q0; j < q1; j++){ dacc_add(v[5][0], v[5][1], a.e2[j]); nd += a.nd[j]; }
     // exclusive scan of the kept counts over the threads' tile ranges, offsets from acc->nkept
     shc[0][threadIdx.x] = nk; __syncthreads();
     for(int d = 1; d < CB; d <<= 1){
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
     }
     __syncthreads();
     for(int k = 0; k < 6; k++){ sh[k][0][threadIdx.x] = v[k][0]; sh[k][1][threadIdx.x] = v[k][1]; }
-    shc[0][threadIdx.x] = nv; shc[1][threadIdx.x] = nsk; shc[2][threadIdx.x] = nk; shc[3][threadIdx.x] = nd;
+    shc[0][threadIdx.x] = nv; shc[1][threadIdx.x] = nsk; shc[2][threadIdx.x] = nk; shc[3][threadIdx.x] = nd; shc[4][threadIdx.x] = nq;
     __syncthreads();
     // fixed pairwise tree over the threads
     for(int d = CB/2; d > 0; d >>= 1){
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
                 dacc_addd(h, l, sh[k][0][o], sh[k][1][o]);
                 sh[k][0][threadIdx.x] = h; sh[k][1][threadIdx.x] = l;
             }
-            for(int k = 0; k < 4; k++) shc[k][threadIdx.x] += shc[k][o];
+            for(int k = 0; k < 5; k++) shc[k][threadIdx.x] += shc[k][o];
         }
         __syncthreads();
     }
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
         dacc_addd(A->b2[0], A->b2[1], sh[3][0][0], sh[3][1][0]);
         dacc_addd(A->drop[0], A->drop[1], sh[4][0][0], sh[4][1][0]);
         dacc_addd(A->u[0], A->u[1], sh[5][0][0], sh[5][1][0]);
-        A->nv += shc[0][0]; A->nskip += shc[1][0]; A->nkept += shc[2][0]; A->nd += shc[3][0]; A->nq += (unsigned long long)m;
+        A->nv += shc[0][0]; A->nskip += shc[1][0]; A->nkept += shc[2][0]; A->nd += shc[3][0]; A->nq += shc[4][0];
     }
 }
 
@@ -431,5 +432,34 @@ extern "C" int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t
     std::vector<double> f(m), a(m);
     if(m && (rc = eval_weighted(graph, h, K, m, kk.data(), f.data(), a.data()))) return rc;
     for(long j = 0; j < m; j++){ fw[at[j]] = f[j]; abw[at[j]] = a[j]; }
+    return SFX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// calibration copy: 16-byte vector loads and stores, grid-stride
+__global__ void __launch_bounds__(256) k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, long n){
+    for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) b[i] = a[i];
+}
+
+extern "C" int sfx_copy_probe(long bytes, double *ms){
+    int rc;
+    if((rc = require_init())) return rc;
+    if(bytes < 16 || bytes % 16){ set_error("sfx_copy_probe: bytes must be a positive multiple of 16"); return SFX_EINVAL; }
+    DevBuf a, b;
+    if((rc = a.reserve(bytes)) || (rc = b.reserve(bytes))){ a.release(); b.release(); return rc; }
+    hipStream_t s = lib().stream;
+    hipEvent_t e0, e1;
+    SFX_HIP(hipEventCreate(&e0)); SFX_HIP(hipEventCreate(&e1));
+    SFX_HIP(hipMemsetAsync(a.p, 1, bytes, s));
+    SFX_HIP(hipEventRecord(e0, s));
+    long n = bytes/16, nb = (n + 255)/256; if(nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)nb), dim3(256), 0, s, a.as<const double2>(), b.as<double2>(), n);
+    SFX_HIP(hipGetLastError());
+    SFX_HIP(hipEventRecord(e1, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    float t = 0; SFX_HIP(hipEventElapsedTime(&t, e0, e1));
+    if(ms) *ms = t;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    a.release(); b.release();
     return SFX_OK;
 }

@@ -210,3 +210,56 @@ def graph_value(info, res):
     d2 = abs(S1 - S0) if not math.isnan(S1) else float('nan')
     disc = 2*d2*d2/res['Ah'] if res['Ah'] > 0 else 0.0
     return w*S0, w*(disc + res['prune'] + res['trunc'] + res['unres'])
+
+
+# ---- job engine (continuous batching) ----
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+
+def _engine_sigs():
+    L = lib()
+    if getattr(L, '_engine_sigs', False):
+        return L
+    L.sfx_engine_start.argtypes = [C.c_int, C.c_double]
+    L.sfx_job_submit.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(Opts), C.POINTER(C.c_long)]
+    L.sfx_job_poll.argtypes = [C.c_long, C.POINTER(C.c_int)]
+    L.sfx_job_wait.argtypes = [C.c_long, C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.sfx_job_release.argtypes = [C.c_long]
+    L._engine_sigs = True
+    return L
+
+
+def engine_start(workers=0, hbm_budget_bytes=0.0):
+    _check(_engine_sigs().sfx_engine_start(workers, hbm_budget_bytes))
+
+
+def engine_stop():
+    _check(_engine_sigs().sfx_engine_stop())
+
+
+def submit(graph, h, N, eps, top=None, **kw):
+    """queue one hierarchical sum; returns the job id at once"""
+    L = _engine_sigs()
+    gi = find(graph) if isinstance(graph, str) else graph
+    top = default_top(h) if top is None else top
+    j = C.c_long()
+    o = opts(**kw)
+    _check(L.sfx_job_submit(gi, h, N, eps, top, C.byref(o), C.byref(j)))
+    return j.value
+
+
+def poll(job):
+    s = C.c_int()
+    _check(_engine_sigs().sfx_job_poll(job, C.byref(s)))
+    return s.value
+
+
+def wait(job, release=True):
+    """(result dict, seconds queued, seconds running) of a job, blocking until it is done"""
+    L = _engine_sigs()
+    r = Result(); tq = C.c_double(); tr = C.c_double()
+    rc = L.sfx_job_wait(job, C.byref(r), C.byref(tq), C.byref(tr))
+    if release:
+        L.sfx_job_release(job)
+    _check(rc)
+    return result_dict(r), tq.value, tr.value
