@@ -117,6 +117,15 @@ def roofline(S, opc):
               'achieved': round(ach, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s', 'frac': round(ach/PEAK_FP64, 4),
               'traffic': None, 'launches': top['launches'], 'avg_launch_ms': round(top['ms']/top['launches'], 4),
               'flops_per_launch': fl/top['launches'], 'share_of_device_time': round(top['ms']/tot_ms, 3)}
+    if rl:
+        # HBM bytes per launch of this kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+        # (each counter scaled by what it reads for the library's copy kernel of known size)
+        import glob
+        for p in glob.glob(os.path.join(ROOT, 'profiles', 'traffic_*.json')):
+            t = json.load(open(p))
+            if t.get('kernel') == rl['kernel']:
+                rl['traffic'] = t['traffic_bytes_per_dispatch']
+                rl['traffic_source'] = os.path.relpath(p, ROOT)
     agg = [e for e in ent if e['kind'] == 'level_eval']
     if agg and rl:
         f_all = sum(flops(e) for e in agg)
