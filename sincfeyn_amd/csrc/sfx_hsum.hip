@@ -460,7 +460,7 @@ static int check_job(int graph, double h, int N, double eps, int top){
     return SFX_OK;
 }
 
-namespace sfx { void budget_acquire(double bytes); void budget_release(double bytes); }
+namespace sfx { void budget_acquire(double bytes, double mine); void budget_release(double bytes); }
 
 // hsum of one graph on the context's stream (arguments checked by the caller)
 static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
@@ -503,11 +503,11 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     set_params(J, abserr);
     res->esok = J.D < 12;
     double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
-    budget_acquire(held);
+    budget_acquire(held, 0);
     for(int lev = top - 1; lev >= 0; lev--){
         long nnext = 0; LevelAcc A;
         double want = lev > 0 ? (double)ncur*(double)(1L << J.D)*sizeof(sfx_cell_t) : 0;   // bound of the next list
-        budget_acquire(want);
+        budget_acquire(want, held);
         if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))){ budget_release(held + want); return rc; }
         if(lev > 0){ double now_ = (double)nnext*sizeof(sfx_cell_t); budget_release(held + want - now_); held = now_; }
         double S = A.s[0] + A.s[1];
@@ -552,14 +552,16 @@ struct Budget {
     std::mutex mu; std::condition_variable cv; double cap = 0, used = 0;
 };
 static Budget &budget(){ static Budget B; return B; }
-void budget_acquire(double bytes){
-    Budget &B = budget(); if(B.cap <= 0 || bytes <= 0) return;
+// `mine` is what the caller already holds: a job waits only for the reservations of other jobs, and is
+// admitted when it would be the only holder even if it alone exceeds the budget
+void budget_acquire(double bytes, double mine){
+    Budget &B = budget(); if(bytes <= 0) return;
     std::unique_lock<std::mutex> lk(B.mu);
-    B.cv.wait(lk, [&]{ return B.used <= 0 || B.used + bytes <= B.cap; });
+    B.cv.wait(lk, [&]{ return B.cap <= 0 || B.used - mine <= 0 || B.used + bytes <= B.cap; });
     B.used += bytes;
 }
 void budget_release(double bytes){
-    Budget &B = budget(); if(B.cap <= 0 || bytes <= 0) return;
+    Budget &B = budget(); if(bytes <= 0) return;
     { std::lock_guard<std::mutex> lk(B.mu); B.used -= bytes; if(B.used < 0) B.used = 0; }
     B.cv.notify_all();
 }
@@ -641,6 +643,7 @@ void engine_shutdown(){
     E.ctx.clear();
     std::lock_guard<std::mutex> lk(E.mu);
     E.queue.clear(); E.jobs.clear(); E.running = 0;
+    { std::lock_guard<std::mutex> lb(budget().mu); budget().cap = 0; }   // no budget outside the engine
 }
 
 }  // namespace sfx
