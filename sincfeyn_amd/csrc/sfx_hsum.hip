@@ -1,9 +1,9 @@
 // sfx_hsum.hip -- the hierarchical Sinc sum of one graph (reference driver/sum.c:556-814 with the
 // symmetry fold off, as the reference's GPU build runs it, sum.c:571).
 //
-//  * coarsest level (step 2^top h): the support-following grow (reference driver/sumkey.h:130-461) on the
-//    host, every frontier evaluated on the device (k_eval nodes + ordered fallback list + k_dd); then the
-//    dilation (sumkey.h:20-90) on the host; the cells go to HBM once;
+//  * coarsest level (step 2^top h): the support-following grow (reference driver/sumkey.h:130-461) and the
+//    dilation (sumkey.h:20-90) on the device (sfx_grow.inc), only the order-dependent frontier sums on
+//    the host; the sorted top-level cells are left in HBM for the refinement;
 //  * refinement levels top-1 .. 0: entirely on the device, in blocks of cells -- children made in the
 //    kernel, FP64 evaluation with the per-point tests, the double-double recompute of the flagged points,
 //    deterministic double-double block sums, and the ordered compaction of the kept children into the
@@ -29,7 +29,12 @@ static double now(){ return std::chrono::duration<double>(std::chrono::steady_cl
 
 struct JobBufs {
     DevBuf EX, pts, v, fl, ddblk, ddidx, ddcnt, e2, nd, cur, nxt, part, keepoff, acc;
-    void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc};
+    // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
+    DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
+           dkey, dns, dtr, dflag, skey, skey2, sortt;
+    void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
+                                     &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
+                                     &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &dflag, &skey, &skey2, &sortt};
                     for(DevBuf *b : all) b->release(); }
 };
 // a context: one HIP stream and its device buffers; the library's own context serves sfx_hsum and the
@@ -109,27 +114,30 @@ struct Job {
 // evaluate a list of nodes on the device with the per-point tests and the fallback (leafx semantics,
 // eval.c:33-60): v, flags (SFX_PT_*), plus the fallback count, the round column's e^2 and the
 // unresolved count, accumulated in list order
-static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v, uint8_t *flg, long *nq, double *u, long *nd){
+__global__ void k_grow_shell(const sfx_cell_t *pts, long n, int K, int st, int D, uint8_t *fl);
+
+// nodes already on the device (dpts); the values stay in B.v / B.fl and are copied to v, fl (raw flag
+// bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that step
+static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, double *v, uint8_t *fl, long *nq, double *u, long *nd){
     if(n <= 0) return SFX_OK;
     JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; int rc;
     long nb = (n + TILE - 1)/TILE;
-    if((rc = B.pts.reserve(n*sizeof(sfx_cell_t))) || (rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) ||
+    if((rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) ||
        (rc = B.ddblk.reserve((2*nb + 2)*sizeof(uint32_t))) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
-       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double))) || (rc = B.nd.reserve(n))) return rc;
+       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double)))) return rc;
     hipStream_t s = J.s;
-    SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-    SFX_HIP(J.g->eval(s, 0, B.pts.as<sfx_cell_t>(), n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+    SFX_HIP(J.g->eval(s, 0, dpts, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
-    SFX_HIP(J.g->dd(s, 0, B.pts.as<sfx_cell_t>(), 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
+    SFX_HIP(J.g->dd(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
                     B.v.as<double>(), B.e2.as<double>(), B.fl.as<uint8_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
+    if(st > 0) hipLaunchKernelGGL(k_grow_shell, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, dpts, n, J.K, st, J.D, B.fl.as<uint8_t>());
     uint32_t m = 0;
-    std::vector<uint8_t> fl(n);
     SFX_HIP(hipMemcpyAsync(v, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipMemcpyAsync(fl.data(), B.fl.p, n, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(fl, B.fl.p, n, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipMemcpyAsync(&m, B.ddcnt.p, sizeof m, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
     std::vector<double> e2; std::vector<uint32_t> idx(m);
@@ -140,9 +148,20 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
         SFX_HIP(hipStreamSynchronize(s));
     }
     if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m); }
-    for(long i = 0; i < n; i++) flg[i] = fl[i] & (SFX_FL_NEAR | SFX_FL_DD);
-    for(uint32_t j = 0; j < m; j++){ long i = idx[j]; *u += e2[i]; if(fl[i] & SFX_FL_UNRES){ (*nd)++; flg[i] |= SFX_PT_UNRES; } }
+    for(uint32_t j = 0; j < m; j++){ long i = idx[j]; *u += e2[i]; if(fl[i] & SFX_FL_UNRES) (*nd)++; }
     *nq += m;
+    return SFX_OK;
+}
+
+static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v, uint8_t *flg, long *nq, double *u, long *nd){
+    if(n <= 0) return SFX_OK;
+    JobBufs &B = J.ctx->B; int rc;
+    if((rc = B.pts.reserve(n*sizeof(sfx_cell_t)))) return rc;
+    hipStream_t s = J.s;
+    SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
+    std::vector<uint8_t> fl(n);
+    if((rc = eval_nodes_core(J, B.pts.as<sfx_cell_t>(), n, 0, v, fl.data(), nq, u, nd))) return rc;
+    for(long i = 0; i < n; i++) flg[i] = (fl[i] & (SFX_FL_NEAR | SFX_FL_DD)) | ((fl[i] & SFX_FL_UNRES) ? SFX_PT_UNRES : 0);
     return SFX_OK;
 }
 
@@ -165,144 +184,6 @@ struct GrowOut {
     std::vector<sfx_cell_t> keep;
     double S = 0, A = 0, e1 = 0, e2 = 0, drop = 0, u = 0; long nev = 0, nq = 0, nd = 0, btuns = 0;
 };
-
-// the support-following grow at step st = 2^lev (lev >= 1), reference sumkey.h:130-461: frontier L+1 is the
-// axis neighbours of frontier L's kept points not evaluated before, in the order of a pass over (kept
-// point, axis, sign); a neighbour is known as evaluated when it is a kept point of frontier L-1 or a
-// boundary point of a frontier L' (n <= L' <= L-1, same parity as its norm n) whose table has not been
-// freed.  Every frontier is summed in order (Neumaier for S, plain for the reference scale A); points
-// with |v| >= eps_rel A (running A) are kept.  At the end the final threshold is re-applied and the
-// shadow tiers are flood-filled.
-static int grow(Job &J, int lev, double eps_rel, GrowOut &G){
-    const int D = J.D, K = J.K, st = 1 << lev;
-    Grid gr; if(!gr.init(D, K, st)){ set_error("grow: W^D does not fit 64-bit keys (D=%d, W=%d)", D, 2*(K/st) + 1); return SFX_EUNSUP; }
-    const double pw = pow((double)st, (double)D);
-    std::vector<sfx_cell_t> fr(1), nx; memset(&fr[0], 0, sizeof(sfx_cell_t));
-    std::vector<double> kv; std::vector<uint64_t> pk;           // |v| of kept points; keys of the previous frontier's kept points
-    KeyMap bmap(1 << 16);                                          // boundary point -> its frontier
-    Neu ks; double a = 0, x1 = 0, x2 = 0, dr = 0;
-    int lay = 0; long BTLO = 0;
-    std::vector<double> v; std::vector<uint8_t> flg;
-    while(!fr.empty()){
-        long n = (long)fr.size();
-        v.resize(n); flg.resize(n);
-        int rc = eval_nodes_finished(J, fr.data(), n, v.data(), flg.data(), &G.nq, &G.u, &G.nd);
-        if(rc) return rc;
-        G.nev += n;
-        for(long i = 0; i < n; i++){ ks.add(v[i]*pw); a += fabs(v[i])*pw; }
-        const double thr = eps_rel*a;
-        std::vector<long> kj;
-        for(long i = 0; i < n; i++){
-            int mx = 0; for(int d = 0; d < D; d++){ int x = abs((int)fr[i].k[d]); if(x > mx) mx = x; }
-            double w = fabs(v[i])*pw; bool kp = fabs(v[i]) >= thr;
-            if(mx == K) x1 += w; else if(mx == K - st) x2 += w;
-            if(!kp){ dr += w; *bmap.insert(gr.key(fr[i].k), nullptr) = lay; }
-            else {
-                kj.push_back(i);
-                sfx_cell_t c = fr[i]; c.fl = (J.o.abskip ? (uint8_t)!(flg[i] & SFX_PT_NEAR) : 0);
-                G.keep.push_back(c); kv.push_back(fabs(v[i]));
-            }
-        }
-        // candidates of the next frontier
-        KeyMap hs(pk.size() + kj.size()*2*D);
-        for(uint64_t y : pk) *hs.insert(y, nullptr) = 1;
-        nx.clear();
-        for(long j : kj){
-            const sfx_cell_t &p = fr[j]; uint64_t y0 = gr.key(p.k);
-            int nrm = 0; for(int d = 0; d < D; d++) nrm += abs((int)p.k[d]);
-            nrm /= st;
-            for(int d = 0; d < D; d++) for(int sg = -1; sg <= 1; sg += 2){
-                int kk = p.k[d] + st*sg;
-                if(kk > K || kk < -K) continue;
-                uint64_t y = sg > 0 ? y0 + gr.P[d] : y0 - gr.P[d];
-                int n1 = abs(kk) > abs((int)p.k[d]) ? nrm + 1 : nrm - 1;
-                if(n1 < BTLO) G.btuns++;
-                const int32_t *bl = bmap.find(y);
-                if(bl && *bl >= n1 && *bl < lay && ((*bl - n1) & 1) == 0 && *bl >= BTLO) continue;
-                int fresh; hs.insert(y, &fresh);
-                if(!fresh) continue;
-                sfx_cell_t c = p; c.k[d] = (int8_t)kk; c.fl = 0;
-                nx.push_back(c);
-            }
-        }
-        pk.clear(); for(long j : kj) pk.push_back(gr.key(fr[j].k));
-        if(!nx.empty() && !J.o.keepbt){
-            long mn = 1L << 40;
-            for(const sfx_cell_t &c : nx){ long t = 0; for(int d = 0; d < D; d++) t += abs((int)c.k[d]); t /= st; if(t < mn) mn = t; }
-            mn -= J.o.btmargin;
-            if(mn - 1 > BTLO) BTLO = mn - 1 < lay + 1 ? mn - 1 : lay + 1;
-        }
-        fr.swap(nx); lay++;
-    }
-    // the final threshold (sumkey.h:397-401)
-    const double thr = eps_rel*a;
-    long m = 0;
-    for(size_t i = 0; i < G.keep.size(); i++){
-        if(kv[i] >= thr){ kv[m] = kv[i]; G.keep[m] = G.keep[i]; m++; } else dr += kv[i]*pw;
-    }
-    G.keep.resize(m); kv.resize(m);
-    // shadow tiers t = 1, 2: reached from z = 0 through kept points with |f| >= 10^t thr (sumkey.h:402-453)
-    if(m && D < 12){
-        KeyMap idx(m);
-        for(long i = 0; i < m; i++) *idx.insert(gr.key(G.keep[i].k), nullptr) = (int32_t)i;
-        std::vector<uint8_t> tr(m, 0); std::vector<long> cu, nw;
-        for(int t = 1; t <= 2; t++){
-            double lim = (t == 1 ? 10 : 100)*thr;
-            cu.clear();
-            for(long i = 0; i < m; i++){
-                bool z = true; for(int d = 0; d < D; d++) if(G.keep[i].k[d]){ z = false; break; }
-                if(z && kv[i] >= lim){ tr[i] = (uint8_t)t; cu.push_back(i); }
-            }
-            while(!cu.empty()){
-                nw.clear();
-                for(long i : cu){
-                    uint64_t y0 = gr.key(G.keep[i].k);
-                    for(int d = 0; d < D; d++) for(int sg = -1; sg <= 1; sg += 2){
-                        int kk = G.keep[i].k[d] + st*sg;
-                        if(kk > K || kk < -K) continue;
-                        const int32_t *q = idx.find(sg > 0 ? y0 + gr.P[d] : y0 - gr.P[d]);
-                        if(q && kv[*q] >= lim && tr[*q] < t){ tr[*q] = (uint8_t)t; nw.push_back(*q); }
-                    }
-                }
-                cu.swap(nw);
-            }
-        }
-        for(long i = 0; i < m; i++) G.keep[i].fl = (uint8_t)((G.keep[i].fl & 1) | (tr[i] << 1));
-    }
-    G.S = ks.val(); G.A = a; G.e1 = x1; G.e2 = x2; G.drop = dr;
-    return SFX_OK;
-}
-
-// the axis neighbours of every kept cell at step st, merged (skip bit: AND, tier: max) and sorted in
-// lexicographic order of k (reference sumkey.h:20-90)
-static int dilate(const Job &J, std::vector<sfx_cell_t> &cells, int st){
-    Grid gr; if(!gr.init(J.D, J.K, st)){ set_error("dilate: keys do not fit"); return SFX_EUNSUP; }
-    const int D = J.D, K = J.K;
-    KeyMap m(cells.size()*(2*D + 1));
-    for(const sfx_cell_t &c : cells){
-        int sk = c.fl & 1, tr = c.fl >> 1; uint64_t y0 = gr.key(c.k);
-        for(int j = -1; j < D; j++) for(int sg = -1; sg <= 1; sg += 2){
-            if(j < 0 && sg > 0) continue;
-            uint64_t y = y0;
-            if(j >= 0){ int kk = c.k[j] + st*sg; if(kk > K || kk < -K) continue; y = sg > 0 ? y + gr.P[j] : y - gr.P[j]; }
-            int fresh; int32_t *e = m.insert(y, &fresh);
-            int esk = fresh ? 1 : (*e & 1), etr = fresh ? 0 : (*e >> 1);
-            if(!sk) esk = 0;
-            if(tr > etr) etr = tr;
-            *e = esk | (etr << 1);
-        }
-    }
-    std::vector<uint64_t> keys; keys.reserve(m.n);
-    for(size_t i = 0; i <= m.mask; i++) if(m.key[i]) keys.push_back(m.key[i]);
-    std::sort(keys.begin(), keys.end());
-    cells.resize(keys.size());
-    for(size_t j = 0; j < keys.size(); j++){
-        sfx_cell_t &c = cells[j]; memset(&c, 0, sizeof c);
-        gr.decode(keys[j], c.k);
-        c.fl = (uint8_t)*m.find(keys[j]);
-    }
-    return SFX_OK;
-}
 
 // box_zero (reference sum.c:346-359): z = 0 and 4096 pseudo-random nodes of the step-2^lev lattice; the
 // node sequence is the reference's (a multiplicative hash of the probe number)
@@ -452,6 +333,8 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     return SFX_OK;
 }
 
+#include "sfx_grow.inc"
+
 static int check_job(int graph, double h, int N, double eps, int top){
     if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("bad graph index %d", graph); return SFX_EINVAL; }
     if(top < 1 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("need h > 0, N >= 3, eps > 0, 1 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
@@ -482,7 +365,8 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     if((rc = box_zero(J, top, &zero, &res->nev, &res->unresdrop))) return rc;
     if(zero){ res->zero = 1; res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
     GrowOut G;
-    if((rc = grow(J, top, eps*1e-3, G))) return rc;
+    long ncur = 0;                      // the grow and the dilation leave the sorted top-level cells in B.cur
+    if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc;
     res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
     res->nev_level[top] = G.nev;
     res->ref = G.A; res->prune = G.drop;
@@ -490,14 +374,8 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
     { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
     double Slev[16]; int nl = 0; Slev[nl++] = G.S;
-    std::vector<sfx_cell_t> &cells = G.keep;
-    if((rc = dilate(J, cells, 1 << top))) return rc;
-    res->kept_top = (long)cells.size();
-    double t2 = now(); res->t_dilate = t2 - t1;
-    // refinement on the device
-    long ncur = (long)cells.size();
-    if((rc = B.cur.reserve((size_t)(ncur ? ncur : 1)*sizeof(sfx_cell_t)))) return rc;
-    if(ncur) SFX_HIP(hipMemcpyAsync(B.cur.p, cells.data(), ncur*sizeof(sfx_cell_t), hipMemcpyHostToDevice, J.s));
+    res->kept_top = ncur;
+    double t2 = now(); res->t_dilate = 0;
     double abserr = fmax(J.o.abserr0, J.o.abrel*eps*G.A);          // sum.c:651
     res->abserr = abserr;
     set_params(J, abserr);
