@@ -31,7 +31,9 @@ sys.path.insert(0, ROOT)
 
 H, N, EPS, TOP = 0.7, 139, 1e-12, 2
 PEAK_FP64 = 78.6          # TFLOP/s, MI355X FP64 vector (FMA counted as 2)
-KIND = {0: 'level_eval', 1: 'level_dd', 2: 'level_sum', 3: 'nodes_eval', 4: 'nodes_dd'}
+KIND = {0: 'level_eval', 1: 'level_dd', 2: 'level_sum', 3: 'nodes_eval', 4: 'nodes_dd', 5: 'level_eval_skip',
+        6: 'level_split'}
+KNAME = {'level_eval': 'k_eval<%s, list, AB=1>', 'level_eval_skip': 'k_eval<%s, list, AB=0>'}
 # bounded CPU sample: graphs of the same workload whose reference run takes ~10-30 s on a few cores
 CPU_SAMPLE = ['g3_01', 'g3_07', 'g3_09', 'g3_17', 'g3_19', 'g3_25', 'g3_27', 'v3_00', 'v3_03', 'v3_04', 'v3_06',
               'v3_07', 'v3_p4Sc', 'v3_p4Sm']
@@ -105,7 +107,7 @@ def roofline(S, opc):
 
     def flops(e):
         oc = opc[e['graph']]
-        if e['kind'] in ('level_eval', 'nodes_eval'):
+        if e['kind'] in ('level_eval', 'level_eval_skip', 'nodes_eval'):
             return e['skipped']*oc['ops0'] + (e['points'] - e['skipped'])*oc['ops1']
         return None
     fl = flops(top)
@@ -113,7 +115,7 @@ def roofline(S, opc):
     if fl is not None:
         avg_s = top['ms']/top['launches']*1e-3
         ach = fl/top['launches']/avg_s/1e12
-        rl = {'bound': 'fp64-valu', 'kernel': 'k_eval<%s, cells>' % top['graph'] if top['kind'] == 'level_eval' else top['kind'],
+        rl = {'bound': 'fp64-valu', 'kernel': KNAME[top['kind']] % top['graph'] if top['kind'] in KNAME else top['kind'],
               'achieved': round(ach, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s', 'frac': round(ach/PEAK_FP64, 4),
               'traffic': None, 'launches': top['launches'], 'avg_launch_ms': round(top['ms']/top['launches'], 4),
               'flops_per_launch': fl/top['launches'], 'share_of_device_time': round(top['ms']/tot_ms, 3)}
@@ -126,7 +128,7 @@ def roofline(S, opc):
             if t.get('kernel') == rl['kernel']:
                 rl['traffic'] = t['traffic_bytes_per_dispatch']
                 rl['traffic_source'] = os.path.relpath(p, ROOT)
-    agg = [e for e in ent if e['kind'] == 'level_eval']
+    agg = [e for e in ent if e['kind'] in ('level_eval', 'level_eval_skip')]
     if agg and rl:
         f_all = sum(flops(e) for e in agg)
         ms_all = sum(e['ms'] for e in agg)

@@ -145,11 +145,13 @@ int sfx_job_release(long job);
  * With profiling on, every device kernel the library launches is bracketed by HIP events on the
  * library's stream, and the elapsed times are accumulated per (graph, kernel kind) together with the
  * points it processed (valid lattice points, and how many of them skipped the |.| sums). */
-#define SFX_K_LEVEL_EVAL 0     /* refinement: children of cells, FP64 + tests */
+#define SFX_K_LEVEL_EVAL 0     /* refinement: children of the cells that compute the |.| sums, FP64 + tests */
 #define SFX_K_LEVEL_DD   1     /* refinement: double-double recompute of the flagged children */
 #define SFX_K_LEVEL_SUM  2     /* refinement: compaction, block sums, next-level cells */
 #define SFX_K_NODES_EVAL 3     /* coarsest level: frontier nodes, FP64 + tests */
 #define SFX_K_NODES_DD   4     /* coarsest level: double-double recompute */
+#define SFX_K_LEVEL_EVAL_SKIP 5 /* refinement: children of the cells that skip the |.| sums, FP64 + tests */
+#define SFX_K_LEVEL_SPLIT 6    /* refinement: split of a block's cells into the two classes */
 typedef struct {
     char graph[32]; int kind;
     long launches; double ms; long points, skipped, dd_points;

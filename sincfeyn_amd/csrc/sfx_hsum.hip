@@ -29,18 +29,19 @@ static double now(){ return std::chrono::duration<double>(std::chrono::steady_cl
 
 struct JobBufs {
     DevBuf EX, pts, v, fl, ddblk, ddidx, ddcnt, e2, nd, cur, nxt, part, keepoff, acc;
+    DevBuf ccls, lsk, lfu, lcnt;            // a refinement block's cells split by |.|-sum class
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
     DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
            dkey, dns, dtr, dflag, skey, skey2, sortt;
     void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
-                                     &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
+                                     &ccls, &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
                                      &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &dflag, &skey, &skey2, &sortt};
                     for(DevBuf *b : all) b->release(); }
 };
 // a context: one HIP stream and its device buffers; the library's own context serves sfx_hsum and the
 // per-point calls, each engine worker has its own (so jobs on different workers overlap on the device)
 struct Ctx {
-    hipStream_t s = nullptr; JobBufs B; hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s = nullptr; JobBufs B; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 static Ctx &default_ctx(){ static Ctx C; C.s = lib().stream; return C; }
 
@@ -52,7 +53,7 @@ struct Prof {
 };
 static Prof &prof(){ static Prof P; return P; }
 static int prof_events(Ctx &c){
-    for(int i = 0; i < 4; i++) if(!c.ev[i]) SFX_HIP(hipEventCreate(&c.ev[i]));
+    for(int i = 0; i < 6; i++) if(!c.ev[i]) SFX_HIP(hipEventCreate(&c.ev[i]));
     return SFX_OK;
 }
 static void prof_add(const char *graph, int kind, double ms, long points, long skipped, long dd){
@@ -73,35 +74,6 @@ struct Neu {
     double s = 0, c = 0;
     void add(double x){ double t = s + x; if(fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s; s = t; }
     double val() const { return s + c; }
-};
-
-// open-addressing map from a nonzero 64-bit key to a 32-bit value (linear probing, power-of-two size)
-struct KeyMap {
-    std::vector<uint64_t> key; std::vector<int32_t> val; size_t mask = 0, n = 0;
-    explicit KeyMap(size_t want = 1024){ reset(want); }
-    void reset(size_t want){
-        size_t c = 1024; while(c < 2*want) c <<= 1;
-        key.assign(c, 0); val.assign(c, 0); mask = c - 1; n = 0;
-    }
-    static size_t h(uint64_t y){ return (size_t)((y*0x9E3779B97F4A7C15ULL) >> 21); }
-    void grow(){
-        std::vector<uint64_t> ok; std::vector<int32_t> ov; ok.swap(key); ov.swap(val);
-        size_t c = (mask + 1)*2; key.assign(c, 0); val.assign(c, 0); mask = c - 1; n = 0;
-        for(size_t i = 0; i < ok.size(); i++) if(ok[i]) *insert(ok[i], nullptr) = ov[i];
-    }
-    // slot of y (inserted with value 0 if absent; *fresh = 1 then)
-    int32_t *insert(uint64_t y, int *fresh){
-        if(2*(n + 1) > mask + 1) grow();
-        size_t i = h(y) & mask;
-        while(key[i] && key[i] != y) i = (i + 1) & mask;
-        if(!key[i]){ key[i] = y; val[i] = 0; n++; if(fresh) *fresh = 1; } else if(fresh) *fresh = 0;
-        return &val[i];
-    }
-    const int32_t *find(uint64_t y) const {
-        size_t i = h(y) & mask;
-        while(key[i]){ if(key[i] == y) return &val[i]; i = (i + 1) & mask; }
-        return nullptr;
-    }
 };
 
 struct Job {
@@ -128,7 +100,7 @@ static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, doubl
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-    SFX_HIP(J.g->eval(s, 0, dpts, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     SFX_HIP(J.g->dd(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
@@ -166,20 +138,6 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
 }
 
 // ---------------------------------------------------------------------------------------------------
-// grid keys: the mixed-radix number 1 c_0 .. c_{D-1} in base W = 2 (K/st) + 1, c_i = k_i/st + K/st, so that
-// numeric order is lexicographic order in k (the reference's key, sumkey.h:7-18)
-struct Grid {
-    int D, K, st, W; uint64_t P[SFX_MAXD];
-    bool init(int D_, int K_, int st_){
-        D = D_; K = K_; st = st_; W = 2*(K/st) + 1;
-        unsigned __int128 p = 1;
-        for(int i = D - 1; i >= 0; i--){ P[i] = (uint64_t)p; p *= W; }
-        return p < ((unsigned __int128)1 << 62);
-    }
-    uint64_t key(const int8_t *k) const { uint64_t y = 1; for(int i = 0; i < D; i++) y = y*W + (uint64_t)(k[i]/st + K/st); return y; }
-    void decode(uint64_t y, int8_t *k) const { for(int i = D - 1; i >= 0; i--){ k[i] = (int8_t)(((int)(y % W) - K/st)*st); y /= W; } }
-};
-
 struct GrowOut {
     std::vector<sfx_cell_t> keep;
     double S = 0, A = 0, e1 = 0, e2 = 0, drop = 0, u = 0; long nev = 0, nq = 0, nd = 0, btuns = 0;
@@ -222,6 +180,13 @@ static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s
 
 __global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0) memset(a, 0, sizeof *a); }
 
+// class of each cell of a block: 1 its children skip the |.| sums (ABSKIP and the cell's skip bit,
+// sum.c:402), 2 they compute them
+__global__ void k_cell_class(const sfx_cell_t *cells, long nc, int abskip, uint8_t *cls){
+    long c = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    if(c < nc) cls[c] = (abskip && (cells[c].fl & 1u)) ? 1 : 2;
+}
+
 // one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
 static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
     JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; hipStream_t s = J.s; int rc;
@@ -233,7 +198,8 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     if((rc = B.v.reserve(nmax*sizeof(double))) || (rc = B.fl.reserve(nmax)) || (rc = B.ddblk.reserve((2*nbt + 2)*sizeof(uint32_t))) ||
        (rc = B.ddidx.reserve(nmax*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(nmax*sizeof(double))) ||
        (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
-       (rc = B.acc.reserve(sizeof(LevelAcc)))) return rc;
+       (rc = B.acc.reserve(sizeof(LevelAcc))) || (rc = B.ccls.reserve(Bc)) || (rc = B.lsk.reserve(Bc*sizeof(uint32_t))) ||
+       (rc = B.lfu.reserve(Bc*sizeof(uint32_t))) || (rc = B.lcnt.reserve(64))) return rc;
     hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
     long kept = 0, prof_nv = 0, prof_nsk = 0, prof_nq = 0;
     for(long c0 = 0; c0 < ncur; c0 += Bc){
@@ -241,12 +207,20 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
         const bool pon = prof().on;
         if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-        SFX_HIP(J.g->eval(s, 1, cells, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), 0));
+        // the block's cells in two lists by class, each run by the integrand compiled for its class
+        uint32_t *lc = B.lcnt.as<uint32_t>();
+        hipLaunchKernelGGL(k_cell_class, dim3((unsigned)((nc + 255)/256)), dim3(256), 0, s, cells, nc, J.o.abskip, B.ccls.as<uint8_t>());
+        SFX_HIP(compact_flags(s, B.ccls.as<uint8_t>(), nc, 1, B.ddblk.as<uint32_t>(), B.lsk.as<uint32_t>(), lc));
+        SFX_HIP(compact_flags(s, B.ccls.as<uint8_t>(), nc, 2, B.ddblk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc + 1));
         if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
+        if(J.o.abskip) SFX_HIP(J.g->eval(s, MODE_LIST, 0, cells, B.lsk.as<uint32_t>(), lc, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
+        if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
+        SFX_HIP(J.g->eval(s, MODE_LIST, 1, cells, B.lfu.as<uint32_t>(), lc + 1, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
+        if(pon) SFX_HIP(hipEventRecord(X.ev[3], s));
         SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
         SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P, B.v.as<double>(),
                         B.e2.as<double>(), B.fl.as<uint8_t>()));
-        if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
+        if(pon) SFX_HIP(hipEventRecord(X.ev[4], s));
         // room for every child of this block in the next level's list (kept so far is known exactly)
         if(!last && (rc = reserve_preserving(B.nxt, (size_t)kept*sizeof(sfx_cell_t), (size_t)(kept + n)*sizeof(sfx_cell_t), s))) return rc;
         LevelArgs a;
@@ -255,14 +229,16 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<uint32_t>(); a.acc = B.acc.as<LevelAcc>();
         SFX_HIP(level_reduce(s, a));
         if(pon){
-            SFX_HIP(hipEventRecord(X.ev[3], s));
+            SFX_HIP(hipEventRecord(X.ev[5], s));
             LevelAcc now_; SFX_HIP(hipMemcpyAsync(&now_, B.acc.p, sizeof now_, hipMemcpyDeviceToHost, s));
             SFX_HIP(hipStreamSynchronize(s));
             long dv = (long)now_.nv - prof_nv, dsk = (long)now_.nskip - prof_nsk, dq = (long)now_.nq - prof_nq;
             prof_nv = (long)now_.nv; prof_nsk = (long)now_.nskip; prof_nq = (long)now_.nq;
-            prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(X, 0, 1), dv, dsk, 0);
-            prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(X, 1, 2), dq, 0, dq);
-            prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(X, 2, 3), n, 0, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_SPLIT, prof_ms(X, 0, 1), nc, 0, 0);
+            if(J.o.abskip) prof_add(J.g->name, SFX_K_LEVEL_EVAL_SKIP, prof_ms(X, 1, 2), dsk, dsk, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(X, 2, 3), dv - dsk, 0, 0);
+            prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(X, 3, 4), dq, 0, dq);
+            prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(X, 4, 5), n, 0, 0);
         }
         if(!last){
             unsigned long long k = 0;

@@ -1,18 +1,19 @@
 // sfx_integrand.hpp -- included by every emitted integrand translation unit (k_<graph>.hip).
 //
 // It instantiates, for one graph, the device kernels that evaluate the integrand at lattice points:
-//   k_eval<G, CELLS>  one thread per point: s = e^{k h} from the job's table, the weight w = prod(s_i h)
-//                     in index order (reference driver/lattice.c:30-34), f in FP64, and the reference's
-//                     per-point tests (driver/eval.c:33-53): the value f w (0 if not finite), the "near"
-//                     flag and the fallback flag.  CELLS = 1: point i is child b = i mod 2^D of cell
-//                     c = i / 2^D at k = cell - st b (reference sum.c:405-408); a child below the box is
-//                     not a lattice node (flag VALID clear, value 0).
+//   k_eval<G, MODE, AB>  one thread per point: s = e^{k h} from the job's table, the weight
+//                     w = prod(s_i h) in index order (reference driver/lattice.c:30-34), f in FP64, and the
+//                     reference's per-point tests (driver/eval.c:33-53): the value f w (0 if not finite),
+//                     the "near" flag and the fallback flag.  MODE_LIST: the points are the children
+//                     b = 0 .. 2^D-1 of listed cells c, at k = cell - st b (reference sum.c:405-408); a
+//                     child below the box is not a lattice node (flag VALID clear, value 0).
 //   k_dd<G, CELLS>    one thread per flagged point (an index list): the same point in double-double,
 //                     then the reference's qvalue (eval.c:25-30): value, eps^2 term of the round column,
 //                     and the "unresolved" mark.
 //   k_raw<G>          golden primitive: f and Sum|terms| at nodes, in FP64 (and in DD), without the tests.
-// The FP64 path is chosen per wave: when every lane's point skips the |.| sums the integrand is
-// instantiated with SQ_AB = 0 (those sums are then dead code), else with SQ_AB = 1.
+// The refinement levels run each block of cells as two launches: the cells whose children skip the |.|
+// sums, with the integrand compiled for SQ_AB = 0 (those sums are dead code, far fewer registers), and
+// the others with SQ_AB = 1.  Node lists choose per wave.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <float.h>
@@ -60,23 +61,42 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
     return w;
 }
 
-template<class G, bool CELLS>
+// MODE_NODES: point i is record i.  MODE_LIST: the children of the cells of a list: point i is child
+// b = i mod 2^D of cell c = list[i >> D] of pts, written to slot (c << D) | b -- so a block's cells can be
+// run as two launches (cells whose children skip the |.| sums, and the others) into one value array.
+// AB: 0 every point skips the |.| sums (the integrand is compiled with SQ_AB = 0: those sums are dead
+// code and the register pressure drops), 1 every point computes them, 2 chosen per wave.
+constexpr int MODE_NODES = 0, MODE_LIST = 1;
+
+template<class G, int MODE, int AB>
 __global__ void __launch_bounds__(SFX_BLOCK)
-k_eval(const sfx_cell_t *__restrict__ pts, long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout){
+k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
+       long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout){
     constexpr int D = G::D;
     long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
+    if(blockIdx.x*(long)blockDim.x >= n) return;               // the whole block is past the list
     bool in = i < n;
-    int k[D]; unsigned cfl = 0; bool valid = false;
-    if(in) valid = point_coords<D, CELLS>(pts, i, st, P.K, k, &cfl);
+    int k[D]; unsigned cfl = 0; bool valid = false; long o = i;
+    if(in){
+        if(MODE == MODE_LIST){
+            long c = list[i >> D]; int b = (int)(i & ((1 << D) - 1));
+            o = (c << D) | b;
+            valid = point_coords<D, true>(pts, o, st, P.K, k, &cfl);
+        } else valid = point_coords<D, false>(pts, i, st, P.K, k, &cfl);
+    }
     bool skip = valid && P.abskip && (cfl & 1u);
-    // the |.| sums are skipped only when every lane of the wave may skip them
-    bool allskip = __all(!valid || skip);
     double v = 0; unsigned fl = 0;
     if(valid){
         double sf[D]; double w = lattice_node<D>(P.EX, P.K, P.H, k, sf);
         double ab = 0, f;
-        if(allskip) f = G::template f<double>(sf, &ab, 0);
-        else f = G::template f<double>(sf, &ab, 1);
+        if(AB == 0) f = G::template f<double>(sf, &ab, 0);
+        else if(AB == 1) f = G::template f<double>(sf, &ab, 1);
+        else {
+            // the |.| sums are skipped only when every lane of the wave may skip them
+            if(__all(skip)) f = G::template f<double>(sf, &ab, 0);
+            else f = G::template f<double>(sf, &ab, 1);
+        }
         v = f*w; fl = SFX_FL_VALID;
         if(skip){ fl |= SFX_FL_SKIP; if(!isfinite(v)) v = 0; }
         else {
@@ -86,7 +106,7 @@ k_eval(const sfx_cell_t *__restrict__ pts, long n, int st, sfx_eval_params_t P, 
             else if(!isfinite(v)) v = 0;
         }
     }
-    if(in){ vout[i] = v; flout[i] = (uint8_t)fl; }
+    if(in){ vout[o] = v; flout[o] = (uint8_t)fl; }
 }
 
 // the double-double recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
@@ -142,33 +162,43 @@ k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfi
 }  // namespace sfx
 
 // host-side launchers of one graph, reached through the registry (sfx_registry.h)
+//   eval: mode MODE_NODES, n records of pts (AB chosen per wave); mode MODE_LIST, the children of the
+//         cells list[0 .. *cnt) of pts (cnt a device count; n bounds *cnt << D and sizes the grid), with
+//         AB = 0 or 1 fixed for the launch
 struct sfx_graph_entry {
     const char *name, *volkov; int L, D, weight;
-    hipError_t (*eval)(hipStream_t s, int cells, const sfx_cell_t *pts, long n, int st, const sfx_eval_params_t *P,
-                       double *v, uint8_t *fl, int block);
+    hipError_t (*eval)(hipStream_t s, int mode, int ab, const sfx_cell_t *pts, const uint32_t *list, const uint32_t *cnt,
+                       long n, int st, const sfx_eval_params_t *P, double *v, uint8_t *fl);
     hipError_t (*dd)(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, const uint32_t *cnt,
                      long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
     hipError_t (*raw)(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2, double *o3, double *o4);
-    const void *kfn[4];        // the kernels (eval nodes, eval cells, dd nodes, dd cells), for attribute queries
+    const void *kfn[5];        // the kernels (eval nodes, eval list AB 0, eval list AB 1, dd nodes, dd cells)
 };
+
+#define SFX_EVAL_ARGS const sfx_cell_t*, const uint32_t*, const uint32_t*, long, int, sfx_eval_params_t, double*, uint8_t*
+#define SFX_DD_ARGS const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*
 
 // the host launchers exist only in the host compilation pass; the device pass sees just the kernels
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
-template __global__ void sfx::k_eval<G, false>(const sfx_cell_t*, long, int, sfx_eval_params_t, double*, uint8_t*); \
-template __global__ void sfx::k_eval<G, true>(const sfx_cell_t*, long, int, sfx_eval_params_t, double*, uint8_t*);  \
-template __global__ void sfx::k_dd<G, false>(const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*); \
-template __global__ void sfx::k_dd<G, true>(const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*);  \
+template __global__ void sfx::k_eval<G, sfx::MODE_NODES, 2>(SFX_EVAL_ARGS);                                  \
+template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 0>(SFX_EVAL_ARGS);                                   \
+template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 1>(SFX_EVAL_ARGS);                                   \
+template __global__ void sfx::k_dd<G, false>(SFX_DD_ARGS);                                                   \
+template __global__ void sfx::k_dd<G, true>(SFX_DD_ARGS);                                                    \
 template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, int, double*, double*, double*, double*);
 #else
 #define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
-static hipError_t sfx_eval_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, long n, int st,           \
-                                  const sfx_eval_params_t *P, double *v, uint8_t *fl, int block){            \
+static hipError_t sfx_eval_##NAME(hipStream_t s, int mode, int ab, const sfx_cell_t *pts, const uint32_t *list, \
+                                  const uint32_t *cnt, long n, int st, const sfx_eval_params_t *P, double *v, \
+                                  uint8_t *fl){                                                              \
     if(n <= 0) return hipSuccess;                                                                            \
-    (void)block;                                                                                             \
-    unsigned nb = (unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK);                                                 \
-    if(cells) hipLaunchKernelGGL((sfx::k_eval<G, true>), dim3(nb), dim3(SFX_BLOCK), 0, s, pts, n, st, *P, v, fl); \
-    else hipLaunchKernelGGL((sfx::k_eval<G, false>), dim3(nb), dim3(SFX_BLOCK), 0, s, pts, n, st, *P, v, fl);   \
+    dim3 g((unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK)), b(SFX_BLOCK);                                         \
+    if(mode == sfx::MODE_NODES)                                                                              \
+        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_NODES, 2>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
+    else if(ab == 0)                                                                                         \
+        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 0>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
+    else hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 1>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
     return hipGetLastError();                                                                                \
 }                                                                                                            \
 static hipError_t sfx_dd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, \
@@ -189,6 +219,7 @@ static hipError_t sfx_raw_##NAME(hipStream_t s, long n, const double *sf, int pr
 }                                                                                                            \
 extern "C" const sfx_graph_entry SFX_E_##NAME = {#NAME, VOLKOV, L, D, W,                                      \
     sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME,                                                          \
-    {(const void*)&sfx::k_eval<G, false>, (const void*)&sfx::k_eval<G, true>,                                \
+    {(const void*)&sfx::k_eval<G, sfx::MODE_NODES, 2>, (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 0>,     \
+     (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 1>,                                                        \
      (const void*)&sfx::k_dd<G, false>, (const void*)&sfx::k_dd<G, true>}};
 #endif

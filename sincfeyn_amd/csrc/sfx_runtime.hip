@@ -315,10 +315,10 @@ extern "C" int sfx_graph_info(int i, sfx_graph_info_t *out){
     out->L = g->L; out->D = g->D; out->weight = g->weight;
     if(lib().init){
         hipFuncAttributes fa;
-        if(hipFuncGetAttributes(&fa, g->kfn[1]) == hipSuccess){
+        if(hipFuncGetAttributes(&fa, g->kfn[2]) == hipSuccess){
             out->vgprs = fa.numRegs; out->scratch_bytes = (int)fa.localSizeBytes;
             int occ = 0;
-            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->kfn[1], SFX_BLOCK, 0) == hipSuccess) out->occupancy = occ;
+            if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, g->kfn[2], SFX_BLOCK, 0) == hipSuccess) out->occupancy = occ;
         }
     }
     return SFX_OK;
