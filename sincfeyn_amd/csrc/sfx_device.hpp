@@ -47,7 +47,7 @@ struct LevelAcc {
 };
 
 constexpr int TILE_THREADS = 256;
-constexpr int TILE_ITEMS = 8;                       // items per thread
+constexpr int TILE_ITEMS = 32;                      // items per thread
 constexpr long TILE = (long)TILE_THREADS*TILE_ITEMS;
 
 // ordered compaction: out[0..*count-1] = the indices i < n with (fl[i] & mask) != 0, ascending.

@@ -29,12 +29,12 @@ static double now(){ return std::chrono::duration<double>(std::chrono::steady_cl
 
 struct JobBufs {
     DevBuf EX, pts, v, fl, ddblk, ddidx, ddcnt, e2, nd, cur, nxt, part, keepoff, acc;
-    DevBuf ccls, lsk, lfu, lcnt;            // a refinement block's cells split by |.|-sum class
+    DevBuf lsk, lfu, lcnt;                  // a refinement block's cells split by |.|-sum class; counts
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
     DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
            dkey, dns, dtr, dflag, skey, skey2, sortt;
     void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
-                                     &ccls, &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
+                                     &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
                                      &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &dflag, &skey, &skey2, &sortt};
                     for(DevBuf *b : all) b->release(); }
 };
@@ -100,7 +100,7 @@ static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, doubl
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
+    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), nullptr, nullptr));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     SFX_HIP(J.g->dd(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
@@ -180,12 +180,40 @@ static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s
 
 __global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0) memset(a, 0, sizeof *a); }
 
-// class of each cell of a block: 1 its children skip the |.| sums (ABSKIP and the cell's skip bit,
-// sum.c:402), 2 they compute them
-__global__ void k_cell_class(const sfx_cell_t *cells, long nc, int abskip, uint8_t *cls){
-    long c = blockIdx.x*(long)blockDim.x + threadIdx.x;
-    if(c < nc) cls[c] = (abskip && (cells[c].fl & 1u)) ? 1 : 2;
+// a block's cells split by class: lsk the cells whose children skip the |.| sums (ABSKIP and the
+// cell's skip bit, sum.c:402), lfu the others; cnt[0], cnt[1] count them (zeroed beforehand).  Each
+// workgroup takes SPLIT_ITEMS cells per thread and reserves its ranges with one atomic per list; the
+// lists' order is immaterial (every child is written at its own slot).
+constexpr int SPLIT_ITEMS = 8;
+__device__ __forceinline__ unsigned wave_incl_scan(unsigned x){
+    for(int d = 1; d < 64; d <<= 1){ unsigned y = __shfl_up(x, d, 64); if(__lane_id() >= d) x += y; }
+    return x;
 }
+__global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, long nc, int abskip, uint32_t *lsk,
+                                                    uint32_t *lfu, uint32_t *cnt){
+    __shared__ unsigned ws[4], wf[4], base[2];
+    const long c0 = (long)blockIdx.x*256*SPLIT_ITEMS;
+    unsigned bits = 0, ns = 0, nf = 0;
+    for(int j = 0; j < SPLIT_ITEMS; j++){
+        long c = c0 + j*256 + threadIdx.x;
+        if(c < nc){ bool sk = abskip && (cells[c].fl & 1u); bits |= (sk ? 1u : 2u) << (2*j); ns += sk; nf += !sk; }
+    }
+    unsigned is = wave_incl_scan(ns), jf = wave_incl_scan(nf);
+    int w = threadIdx.x >> 6;
+    if(__lane_id() == 63){ ws[w] = is; wf[w] = jf; }
+    __syncthreads();
+    unsigned os = is - ns, of = jf - nf, ts = 0, tf = 0;
+    for(int v = 0; v < 4; v++){ if(v < w){ os += ws[v]; of += wf[v]; } ts += ws[v]; tf += wf[v]; }
+    if(threadIdx.x == 0){ base[0] = ts ? atomicAdd(&cnt[0], ts) : 0; base[1] = tf ? atomicAdd(&cnt[1], tf) : 0; }
+    __syncthreads();
+    os += base[0]; of += base[1];
+    for(int j = 0; j < SPLIT_ITEMS; j++){
+        unsigned b = (bits >> (2*j)) & 3u;
+        uint32_t c = (uint32_t)(c0 + j*256 + threadIdx.x);
+        if(b == 1u) lsk[os++] = c; else if(b == 2u) lfu[of++] = c;
+    }
+}
+__global__ void k_zero_u32(uint32_t *p, int n){ if(blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0; }
 
 // one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
 static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
@@ -198,7 +226,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     if((rc = B.v.reserve(nmax*sizeof(double))) || (rc = B.fl.reserve(nmax)) || (rc = B.ddblk.reserve((2*nbt + 2)*sizeof(uint32_t))) ||
        (rc = B.ddidx.reserve(nmax*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(nmax*sizeof(double))) ||
        (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
-       (rc = B.acc.reserve(sizeof(LevelAcc))) || (rc = B.ccls.reserve(Bc)) || (rc = B.lsk.reserve(Bc*sizeof(uint32_t))) ||
+       (rc = B.acc.reserve(sizeof(LevelAcc))) || (rc = B.lsk.reserve(Bc*sizeof(uint32_t))) ||
        (rc = B.lfu.reserve(Bc*sizeof(uint32_t))) || (rc = B.lcnt.reserve(64))) return rc;
     hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
     long kept = 0, prof_nv = 0, prof_nsk = 0, prof_nq = 0;
@@ -207,18 +235,20 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
         const bool pon = prof().on;
         if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-        // the block's cells in two lists by class, each run by the integrand compiled for its class
+        // the block's cells in two lists by class, each run by the integrand compiled for its class; the
+        // evaluations append the slots that need the double-double recompute to ddidx
         uint32_t *lc = B.lcnt.as<uint32_t>();
-        hipLaunchKernelGGL(k_cell_class, dim3((unsigned)((nc + 255)/256)), dim3(256), 0, s, cells, nc, J.o.abskip, B.ccls.as<uint8_t>());
-        SFX_HIP(compact_flags(s, B.ccls.as<uint8_t>(), nc, 1, B.ddblk.as<uint32_t>(), B.lsk.as<uint32_t>(), lc));
-        SFX_HIP(compact_flags(s, B.ccls.as<uint8_t>(), nc, 2, B.ddblk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc + 1));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, lc, 3);
+        hipLaunchKernelGGL(k_cell_split, dim3((unsigned)((nc + 256*SPLIT_ITEMS - 1)/(256*SPLIT_ITEMS))), dim3(256), 0, s, cells, nc, J.o.abskip,
+                           B.lsk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc);
         if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
-        if(J.o.abskip) SFX_HIP(J.g->eval(s, MODE_LIST, 0, cells, B.lsk.as<uint32_t>(), lc, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
+        if(J.o.abskip) SFX_HIP(J.g->eval(s, MODE_LIST, 0, cells, B.lsk.as<uint32_t>(), lc, n, st, &J.P, B.v.as<double>(),
+                                         B.fl.as<uint8_t>(), B.ddidx.as<uint32_t>(), lc + 2));
         if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
-        SFX_HIP(J.g->eval(s, MODE_LIST, 1, cells, B.lfu.as<uint32_t>(), lc + 1, n, st, &J.P, B.v.as<double>(), B.fl.as<uint8_t>()));
+        SFX_HIP(J.g->eval(s, MODE_LIST, 1, cells, B.lfu.as<uint32_t>(), lc + 1, n, st, &J.P, B.v.as<double>(),
+                          B.fl.as<uint8_t>(), B.ddidx.as<uint32_t>(), lc + 2));
         if(pon) SFX_HIP(hipEventRecord(X.ev[3], s));
-        SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
-        SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P, B.v.as<double>(),
+        SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), lc + 2, n, &J.P, B.v.as<double>(),
                         B.e2.as<double>(), B.fl.as<uint8_t>()));
         if(pon) SFX_HIP(hipEventRecord(X.ev[4], s));
         // room for every child of this block in the next level's list (kept so far is known exactly)

@@ -23,7 +23,12 @@
 
 #define SFX_FN __device__ __forceinline__
 #ifndef SFX_CHUNK
-#define SFX_CHUNK __device__ __noinline__     // the forest-block functions of the largest integrands
+#define SFX_CHUNK __device__ __noinline__     // the out-of-line forest-block functions of the largest integrands
+#endif
+// forest blocks are inlined into the FP64 kernels (their inputs and outputs then stay in registers instead
+// of a scratch call frame) and called out of line from the double-double kernels (compile time)
+#ifndef SFX_CHUNK_INLINE
+#define SFX_CHUNK_INLINE(R) (sizeof(R) == sizeof(double))
 #endif
 
 #ifndef SFX_BLOCK
@@ -63,7 +68,8 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
 
 // MODE_NODES: point i is record i.  MODE_LIST: the children of the cells of a list: point i is child
 // b = i mod 2^D of cell c = list[i >> D] of pts, written to slot (c << D) | b -- so a block's cells can be
-// run as two launches (cells whose children skip the |.| sums, and the others) into one value array.
+// run as two launches (cells whose children skip the |.| sums, and the others) into one value array;
+// the slots flagged for the double-double recompute are appended to ddidx[0 .. *ddcnt).
 // AB: 0 every point skips the |.| sums (the integrand is compiled with SQ_AB = 0: those sums are dead
 // code and the register pressure drops), 1 every point computes them, 2 chosen per wave.
 constexpr int MODE_NODES = 0, MODE_LIST = 1;
@@ -71,7 +77,8 @@ constexpr int MODE_NODES = 0, MODE_LIST = 1;
 template<class G, int MODE, int AB>
 __global__ void __launch_bounds__(SFX_BLOCK)
 k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
-       long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout){
+       long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout,
+       uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
     constexpr int D = G::D;
     long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
     if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
@@ -107,6 +114,16 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
         }
     }
     if(in){ vout[o] = v; flout[o] = (uint8_t)fl; }
+    if(MODE == MODE_LIST && ddidx){
+        // the points that need the double-double recompute, appended to ddidx (one atomic per wave; the
+        // list's order is immaterial: the recompute writes each point at its own slot)
+        bool q = (fl & SFX_FL_DD) != 0;
+        unsigned long long m = __ballot(q);
+        unsigned base = 0;
+        if(m && __lane_id() == 0) base = atomicAdd(ddcnt, (unsigned)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if(q) ddidx[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = (uint32_t)o;
+    }
 }
 
 // the double-double recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
@@ -168,14 +185,16 @@ k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfi
 struct sfx_graph_entry {
     const char *name, *volkov; int L, D, weight;
     hipError_t (*eval)(hipStream_t s, int mode, int ab, const sfx_cell_t *pts, const uint32_t *list, const uint32_t *cnt,
-                       long n, int st, const sfx_eval_params_t *P, double *v, uint8_t *fl);
+                       long n, int st, const sfx_eval_params_t *P, double *v, uint8_t *fl, uint32_t *ddidx,
+                       uint32_t *ddcnt);
     hipError_t (*dd)(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, const uint32_t *cnt,
                      long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
     hipError_t (*raw)(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2, double *o3, double *o4);
     const void *kfn[5];        // the kernels (eval nodes, eval list AB 0, eval list AB 1, dd nodes, dd cells)
 };
 
-#define SFX_EVAL_ARGS const sfx_cell_t*, const uint32_t*, const uint32_t*, long, int, sfx_eval_params_t, double*, uint8_t*
+#define SFX_EVAL_ARGS const sfx_cell_t*, const uint32_t*, const uint32_t*, long, int, sfx_eval_params_t, double*, uint8_t*, \
+                      uint32_t*, uint32_t*
 #define SFX_DD_ARGS const sfx_cell_t*, int, const uint32_t*, const uint32_t*, sfx_eval_params_t, double*, double*, uint8_t*
 
 // the host launchers exist only in the host compilation pass; the device pass sees just the kernels
@@ -191,14 +210,14 @@ template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, i
 #define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
 static hipError_t sfx_eval_##NAME(hipStream_t s, int mode, int ab, const sfx_cell_t *pts, const uint32_t *list, \
                                   const uint32_t *cnt, long n, int st, const sfx_eval_params_t *P, double *v, \
-                                  uint8_t *fl){                                                              \
+                                  uint8_t *fl, uint32_t *ddidx, uint32_t *ddcnt){                            \
     if(n <= 0) return hipSuccess;                                                                            \
     dim3 g((unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK)), b(SFX_BLOCK);                                         \
     if(mode == sfx::MODE_NODES)                                                                              \
-        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_NODES, 2>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
+        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_NODES, 2>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
     else if(ab == 0)                                                                                         \
-        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 0>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
-    else hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 1>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl); \
+        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 0>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
+    else hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 1>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
     return hipGetLastError();                                                                                \
 }                                                                                                            \
 static hipError_t sfx_dd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, \

@@ -13,7 +13,10 @@ paths come from the same text, and the double path is the arithmetic of the refe
 
 Transforms (text level; the kernels are regular machine-written C):
   * `static REAL FN(name)(params){`    -> `template<class REAL> SFX_FN REAL name(params, const int SQ_AB){`
-  * `static void FN(bk_x)(const REAL *F, REAL *T){` (chunk) -> `template<class REAL> SFX_CHUNK void bk_x(...)`
+  * `static void FN(bk_x)(const REAL *F, REAL *T){` (chunk) -> the body `bk_x_b`, an out-of-line copy
+    `bk_x_c` (SFX_CHUNK) and a dispatcher `bk_x` that inlines the body where SFX_CHUNK_INLINE(REAL) holds
+    (the FP64 kernels: no call frame, F and T stay in registers) and calls the copy otherwise (the
+    double-double kernels, whose inlined bodies would take the compiler many minutes)
   * every call `FN(name)(args)`         -> `name<REAL>(args, SQ_AB)`  (chunk calls get no SQ_AB)
   * the Gauss-Legendre tables at file top -> `__constant__` arrays
   * SQ_AB (the reference's thread-local "compute the |.| sums" flag, common.h:24-26) becomes a parameter;
@@ -113,8 +116,17 @@ def transform(body):
         params = body[m.end():close]
         res.append(body[pos:m.start()])
         if name in chunk:
-            params = params.replace('const REAL *F', 'const REAL *__restrict__ F').replace('REAL *T', 'REAL *__restrict__ T')
-            res.append('template<class REAL> SFX_CHUNK %s %s(%s' % (ret, name, params))
+            if params.strip() != 'const REAL *F, REAL *T':
+                raise EmitError('chunk %s has parameters (%s), not (const REAL *F, REAL *T)' % (name, params))
+            pr = 'const REAL *__restrict__ F, REAL *__restrict__ T'
+            # the body (always inlined into its caller), an out-of-line copy, and the dispatcher the point
+            # function calls: inline where SFX_CHUNK_INLINE(REAL) holds (FP64), else the out-of-line call
+            res.append('template<class REAL> SFX_FN void %s_b(%s);\n' % (name, pr))
+            res.append('template<class REAL> SFX_CHUNK void %s_c(%s){ %s_b<REAL>(F, T); }\n' % (name, pr, name))
+            res.append('template<class REAL> SFX_FN void %s(%s){\n'
+                       '  if constexpr(SFX_CHUNK_INLINE(REAL)) %s_b<REAL>(F, T); else %s_c<REAL>(F, T); }\n'
+                       % (name, pr, name, name))
+            res.append('template<class REAL> SFX_FN %s %s_b(%s' % (ret, name, pr))
         else:
             res.append('template<class REAL> SFX_FN %s %s(%s, const int SQ_AB' % (ret, name, params))
         pos = close

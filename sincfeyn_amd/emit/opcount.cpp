@@ -11,6 +11,7 @@
 #define __constant__
 #define SFX_FN inline
 #define SFX_CHUNK inline
+#define SFX_CHUNK_INLINE(R) true
 static long long N_ADD, N_MUL, N_DIV;
 struct oc {
     double v;

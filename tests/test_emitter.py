@@ -35,7 +35,11 @@ def test_transform_synthetic(tmp_path):
     inc = (tmp_path / 'k_t.inc').read_text()
     hip = (tmp_path / 'k_t.hip').read_text()
     assert meta['name'] == 't' and meta['D'] == 2 and meta['L'] == 2 and meta['volkov'] == '1; 1-2'
-    assert 'template<class REAL> SFX_CHUNK void bk_t_0(const REAL *__restrict__ F, REAL *__restrict__ T)' in inc
+    # a chunk: body bk_t_0_b, out-of-line copy bk_t_0_c, dispatcher bk_t_0 (inline for FP64)
+    assert 'template<class REAL> SFX_FN void bk_t_0_b(const REAL *__restrict__ F, REAL *__restrict__ T){' in inc
+    assert 'template<class REAL> SFX_CHUNK void bk_t_0_c(const REAL *__restrict__ F, REAL *__restrict__ T)' \
+           '{ bk_t_0_b<REAL>(F, T); }' in inc
+    assert 'if constexpr(SFX_CHUNK_INLINE(REAL)) bk_t_0_b<REAL>(F, T); else bk_t_0_c<REAL>(F, T); }' in inc
     assert 'template<class REAL> SFX_FN REAL pt_t(const REAL *x, const REAL Vx, REAL *ab, const int SQ_AB)' in inc
     assert 'bk_t_0<REAL>(F_, T_)' in inc                       # chunk call: no SQ_AB argument
     assert 'pt_t<REAL>(s,0,&a, SQ_AB)' in inc                  # SQ_AB passed on
