@@ -146,6 +146,8 @@ def main():
     ap.add_argument('--graphs', default='all3', help='selector (default all3); a subset is not the benchmark')
     ap.add_argument('--profile-out', default=None, help='write the per-kernel table as JSON here')
     ap.add_argument('--workers', type=int, default=16, help='engine workers of the batched path')
+    ap.add_argument('--no-batched', action='store_true',
+                    help='run the per-graph pass only (one host thread: the form profiled with rocprofv3)')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
     a = ap.parse_args()
@@ -179,6 +181,12 @@ def main():
     if a.profile_out:
         with open(a.profile_out, 'w') as f:
             json.dump(ent, f, indent=1)
+    if a.no_batched:
+        print(json.dumps({'metric': 'Sinc-sum points/s, per-graph pass only (profiling form)', 'value': ev_pg/dt_pg,
+                          'unit': 'points/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+                          'ms_per_step': dt_pg*1e3/a.steps, 'higher_is_better': True, 'path': 'per-graph',
+                          'roofline': rl, 'device': info}))
+        return
     # the batched path: the same jobs through the engine (headline value)
     S.engine_start(a.workers, 0)
     order = sorted(graphs, key=lambda g: -S.graphs()[g[0]]['D'])      # the widest lattices first

@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, lon
         if(b == 1u) lsk[os++] = c; else if(b == 2u) lfu[of++] = c;
     }
 }
-__global__ void k_zero_u32(uint32_t *p, int n){ if(blockIdx.x == 0 && (int)threadIdx.x < n) p[threadIdx.x] = 0; }
+__global__ void k_zero_u32(uint32_t *p, long n){ for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) p[i] = 0; }
 
 // one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
 static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
@@ -227,9 +227,16 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
        (rc = B.ddidx.reserve(nmax*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(nmax*sizeof(double))) ||
        (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
        (rc = B.acc.reserve(sizeof(LevelAcc))) || (rc = B.lsk.reserve(Bc*sizeof(uint32_t))) ||
-       (rc = B.lfu.reserve(Bc*sizeof(uint32_t))) || (rc = B.lcnt.reserve(64))) return rc;
+       (rc = B.lfu.reserve(Bc*sizeof(uint32_t)))) return rc;
+    // the next level's list at its bound (every child kept; the job's HBM reservation covers it), so the
+    // blocks run back to back without a host round trip; per block three counters (two class lists and
+    // the double-double list), all zeroed here
+    const long nblk = (ncur + Bc - 1)/Bc;
+    if(!last && (rc = B.nxt.reserve((size_t)(ncur << D)*sizeof(sfx_cell_t)))) return rc;
+    if((rc = B.lcnt.reserve((size_t)nblk*4*sizeof(uint32_t)))) return rc;
     hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
-    long kept = 0, prof_nv = 0, prof_nsk = 0, prof_nq = 0;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<long>((nblk*4 + 255)/256, 1024)), dim3(256), 0, s, B.lcnt.as<uint32_t>(), nblk*4);
+    long prof_nv = 0, prof_nsk = 0, prof_nq = 0;
     for(long c0 = 0; c0 < ncur; c0 += Bc){
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
@@ -237,8 +244,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
         // the block's cells in two lists by class, each run by the integrand compiled for its class; the
         // evaluations append the slots that need the double-double recompute to ddidx
-        uint32_t *lc = B.lcnt.as<uint32_t>();
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, lc, 3);
+        uint32_t *lc = B.lcnt.as<uint32_t>() + 4*(c0/Bc);
         hipLaunchKernelGGL(k_cell_split, dim3((unsigned)((nc + 256*SPLIT_ITEMS - 1)/(256*SPLIT_ITEMS))), dim3(256), 0, s, cells, nc, J.o.abskip,
                            B.lsk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc);
         if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
@@ -251,8 +257,6 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), lc + 2, n, &J.P, B.v.as<double>(),
                         B.e2.as<double>(), B.fl.as<uint8_t>()));
         if(pon) SFX_HIP(hipEventRecord(X.ev[4], s));
-        // room for every child of this block in the next level's list (kept so far is known exactly)
-        if(!last && (rc = reserve_preserving(B.nxt, (size_t)kept*sizeof(sfx_cell_t), (size_t)(kept + n)*sizeof(sfx_cell_t), s))) return rc;
         LevelArgs a;
         a.v = B.v.as<double>(); a.fl = B.fl.as<uint8_t>(); a.cells = cells; a.n = n; a.D = D; a.st = st; a.last = last;
         a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
@@ -269,12 +273,6 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
             prof_add(J.g->name, SFX_K_LEVEL_EVAL, prof_ms(X, 2, 3), dv - dsk, 0, 0);
             prof_add(J.g->name, SFX_K_LEVEL_DD, prof_ms(X, 3, 4), dq, 0, dq);
             prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(X, 4, 5), n, 0, 0);
-        }
-        if(!last){
-            unsigned long long k = 0;
-            SFX_HIP(hipMemcpyAsync(&k, (char*)B.acc.p + offsetof(LevelAcc, nkept), sizeof k, hipMemcpyDeviceToHost, s));
-            SFX_HIP(hipStreamSynchronize(s));
-            kept = (long)k;
         }
         if(J.o.verbose) fprintf(stderr, "  level %d: cells %ld/%ld\n", lev, c0 + nc, ncur);
     }

@@ -27,8 +27,11 @@ int require_init(){
 
 int DevBuf::reserve(size_t bytes){
     if(bytes <= cap) return SFX_OK;
+    // small buffers double (amortised growth); large ones are sized to the request in 2 MiB steps, so a
+    // level's bound-sized list does not take twice its reservation
     size_t want = cap ? cap : 4096;
-    while(want < bytes) want *= 2;
+    while(want < bytes && want < ((size_t)64 << 20)) want *= 2;
+    if(want < bytes) want = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
     if(p){ hipError_t e = hipFree(p); p = nullptr; cap = 0; if(e != hipSuccess) return hip_fail(e, "hipFree", __FILE__, __LINE__); }
     hipError_t e = hipMalloc(&p, want);
     if(e != hipSuccess){ p = nullptr; set_error("hipMalloc(%zu bytes): %s", want, hipGetErrorString(e)); return SFX_ENOMEM; }
