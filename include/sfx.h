@@ -102,6 +102,25 @@ int sfx_eval_nodes(int graph, double h, int K, long n, const int16_t *k, const u
                    double *fw, double *abw);
 int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t *cells, const uint8_t *skip, int st,
                    double *fw, double *abw);
+
+/* Scale test of a graph's integrand: replaces the reference's `sincqed scale` (driver/main.c:77-116,
+ * scale_test).  From a point x0 (srand(seed); x0_i = exp(4 rand()/RAND_MAX - 2); x0_fixed = 1) every
+ * direction e in {-E..E}^(n-1) \ {0} (the fixed line excluded; direction index d = mixed radix 2E+1,
+ * first free line fastest) scales x_i -> x0_i exp(-k e_i) at k = kappa/3, 2 kappa/3, kappa, and
+ * |g| = |pt(x) prod_{i != fixed} x_i| is taken in double-double (the point function on the device).
+ * A direction is slow when |g(kappa)| / |g(2 kappa/3)| > 0.3 and |g(kappa)| > 1e-30.  SFX_EUNSUP for
+ * the VP graphs, which have no point function. */
+#define SFX_SCALE_MAXSLOW 12
+typedef struct {
+    int n, fixed;                        /* lines of the graph, the fixed line */
+    long ndir;                           /* directions tested, (2E+1)^(n-1) - 1 */
+    int bad;                             /* slow directions */
+    double worst; long worstd;           /* largest ratio and its direction index (-1: none above 0) */
+    int e_worst[16]; double g_worst[3];  /* its exponents (all n lines) and |g| at the three kappas */
+    int nslow;                           /* the first min(bad, 12) slow directions, in index order */
+    int e_slow[SFX_SCALE_MAXSLOW][16]; double g_slow[SFX_SCALE_MAXSLOW][3];
+} sfx_scale_result_t;
+int sfx_scale_test(int graph, double kappa, unsigned seed, int E, sfx_scale_result_t *out);
 /* the integrand at given Sinc variables sf[i*D + d], no weight and no tests.  prec 0: f and Sum|terms|
  * in FP64 into out[2i], out[2i+1]; prec 1: double-double, out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
 int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out);

@@ -10,6 +10,7 @@
  *   driver/sum.c:18-20,76-81 Neumaier sums kadd/kval/kblock/kflush
  *   driver/lattice.c:13-34  the EX table and lattice_node()
  *   driver/main.c:46-48     value and error of a graph
+ *   driver/main.c:79-116    scale_test() (the integrand's point function in double-double)
  * The integrand comes from the reference's own generated kernels, compiled where they lie into
  * oracle/_ref/libsqk_ref.so (graph_t descriptors G_<name>: fd in double, fq in double-double).
  */
@@ -44,6 +45,14 @@ int  orc_eval_fd(const char *name, long n, const double *sf, double *f, double *
 /* fq (double-double half) -> out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
 int  orc_eval_fq(const char *name, long n, const double *sf, double *out);
 int  orc_hsum(const char *name, double h, int N, double eps, int top, const orc_opts_t *o, orc_res_t *r);
+/* the scale test, driver/main.c:79-116 (directions in index order, one thread); -2 for a graph without
+ * a point function (VP) */
+typedef struct {
+    int n, fixed; long ndir; int bad; double worst; long worstd;
+    int e_worst[16]; double g_worst[3];
+    int nslow; int e_slow[12][16]; double g_slow[12][3];
+} orc_scale_t;
+int  orc_scale(const char *name, double kappa, unsigned seed, int E, orc_scale_t *out);
 const char *orc_error(void);
 
 #ifdef __cplusplus

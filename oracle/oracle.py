@@ -37,6 +37,19 @@ class OrcGraph(C.Structure):
     _fields_ = [('volkov', C.c_char * 64), ('L', C.c_int), ('D', C.c_int), ('weight', C.c_int), ('has_fv', C.c_int)]
 
 
+class OrcScale(C.Structure):
+    _fields_ = [('n', C.c_int), ('fixed', C.c_int), ('ndir', C.c_long), ('bad', C.c_int), ('worst', C.c_double),
+                ('worstd', C.c_long), ('e_worst', C.c_int*16), ('g_worst', C.c_double*3), ('nslow', C.c_int),
+                ('e_slow', (C.c_int*16)*12), ('g_slow', (C.c_double*3)*12)]
+
+
+def scale_dict(r):
+    n = r.n
+    return dict(n=n, fixed=r.fixed, ndir=r.ndir, bad=r.bad, worst=r.worst, worstd=r.worstd,
+                e_worst=list(r.e_worst)[:n], g_worst=list(r.g_worst),
+                slow=[(list(r.e_slow[k])[:n], list(r.g_slow[k])) for k in range(r.nslow)])
+
+
 def available():
     return os.path.exists(LIB_ORACLE) and os.path.exists(LIB_KERNELS)
 
@@ -57,6 +70,7 @@ class Oracle:
         L.orc_eval_fq.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p]
         L.orc_hsum.argtypes = [C.c_char_p, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(OrcOpts), C.POINTER(OrcRes)]
         L.orc_opts_default.argtypes = [C.POINTER(OrcOpts)]
+        L.orc_scale.argtypes = [C.c_char_p, C.c_double, C.c_uint, C.c_int, C.POINTER(OrcScale)]
         L.orc_error.restype = C.c_char_p
         self._select()
 
@@ -104,6 +118,18 @@ class Oracle:
         if self.lib.orc_hsum(name.encode(), h, N, eps, top, C.byref(o), C.byref(r)) != 0:
             raise RuntimeError(self.lib.orc_error().decode())
         return res_dict(r)
+
+
+    def scale(self, name, kappa=24.0, seed=1, E=1):
+        """the reference's scale test (main.c:79-116) on the selected kernel set; None for VP graphs"""
+        self._select()
+        r = OrcScale()
+        rc = self.lib.orc_scale(name.encode(), kappa, seed, E, C.byref(r))
+        if rc == -2:
+            return None
+        if rc:
+            raise RuntimeError(self.lib.orc_error().decode())
+        return scale_dict(r)
 
 
 def res_dict(r):

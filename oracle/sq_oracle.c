@@ -63,6 +63,60 @@ int orc_eval_fq(const char *name, long n, const double *sf, double *out){
     return 0;
 }
 
+/* ---- the scale test (main.c:79-116) ---- */
+/* dd.h:18-20,38-40: the product of two double-double values (fma two-product, then renormalised) */
+static ddv dmul(ddv a, ddv b){
+    double p = a.hi*b.hi, e = fma(a.hi, b.hi, -p);
+    e += a.hi*b.lo + a.lo*b.hi;
+    double s = p + e; ddv r = {s, e - (s - p)}; return r;
+}
+typedef ddv (*ptq_t)(const ddv*, ddv, ddv*);
+int orc_scale(const char *name, double kap, unsigned seed, int E, orc_scale_t *out){
+    const graph_t *g = findg(name); if(!g) return -1;
+    if(!g->ptq){ snprintf(ERR, sizeof ERR, "%s has no point function", name); return -2; }
+    int n = g->n, fixed = g->fixed, B = 2*E + 1;
+    memset(out, 0, sizeof *out); out->n = n; out->fixed = fixed; out->worstd = -1;
+    srand(seed);
+    ddv x0[16];
+    for(int i = 0; i < n; i++){ x0[i].hi = exp(4.0*rand()/RAND_MAX - 2.0); x0[i].lo = 0; }   /* main.c:82 */
+    x0[fixed].hi = 1; x0[fixed].lo = 0;
+    long ndir = 1; for(int i = 0; i < n - 1; i++) ndir *= B;
+    out->ndir = ndir - 1;
+    for(long d = 0; d < ndir; d++){
+        int e[16], nz = 0; long r = d;
+        for(int i = 0; i < n; i++){ if(i == fixed){ e[i] = 0; continue; } e[i] = (int)(r % B) - E; r /= B; nz += e[i] != 0; }
+        if(!nz) continue;
+        double gv[3];
+        for(int q = 0; q < 3; q++){
+            double kk = kap*(q + 1)/3.0; ddv x[16], meas = {1, 0};
+            for(int i = 0; i < n; i++){
+                ddv c = {exp(-kk*e[i]), 0};
+                x[i] = dmul(x0[i], c);
+                if(i != fixed) meas = dmul(meas, x[i]);
+            }
+            ddv ab, zero = {0, 0};
+            ddv v = ((ptq_t)g->ptq)(x, zero, &ab);
+            ddv vm = dmul(v, meas);
+            gv[q] = fabs(vm.hi + vm.lo);
+        }
+        double ratio = gv[1] > 0 ? gv[2]/gv[1] : 0;
+        if(ratio > out->worst){
+            out->worst = ratio; out->worstd = d;
+            for(int q = 0; q < 3; q++) out->g_worst[q] = gv[q];
+            for(int i = 0; i < n; i++) out->e_worst[i] = e[i];
+        }
+        if(ratio > 0.3 && gv[2] > 1e-30){
+            if(out->nslow < 12){
+                for(int q = 0; q < 3; q++) out->g_slow[out->nslow][q] = gv[q];
+                for(int i = 0; i < n; i++) out->e_slow[out->nslow][i] = e[i];
+                out->nslow++;
+            }
+            out->bad++;
+        }
+    }
+    return 0;
+}
+
 /* ---- state of one run ---- */
 static const graph_t *G; static int D, K; static double H, ABSERR; static double *EX; static orc_opts_t O;
 static long NQ, UNRESDROP; static double U;

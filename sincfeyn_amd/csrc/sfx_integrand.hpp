@@ -176,6 +176,21 @@ k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfi
     }
 }
 
+// the graph's point function (the reference's ptq, common.h:38) in double-double at n points of the
+// line parameters x (NL values each, hi and lo parts): the scale test's primitive (main.c:79-116)
+template<class G>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_pt(long n, const double *__restrict__ xhi, const double *__restrict__ xlo, double *vhi, double *vlo){
+    if constexpr(G::HAS_PT){
+        long i = blockIdx.x*(long)blockDim.x + threadIdx.x; if(i >= n) return;
+        sfx_dd x[G::NL];
+#pragma unroll
+        for(int j = 0; j < G::NL; j++) x[j] = sfx_dd(xhi[i*G::NL + j], xlo[i*G::NL + j]);
+        sfx_dd ab; sfx_dd v = G::template pt<sfx_dd>(x, &ab);
+        vhi[i] = v.hi; vlo[i] = v.lo;
+    }
+}
+
 }  // namespace sfx
 
 // host-side launchers of one graph, reached through the registry (sfx_registry.h)
@@ -190,6 +205,8 @@ struct sfx_graph_entry {
     hipError_t (*dd)(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, const uint32_t *cnt,
                      long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
     hipError_t (*raw)(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2, double *o3, double *o4);
+    hipError_t (*pt)(hipStream_t s, long n, const double *xhi, const double *xlo, double *vhi, double *vlo);
+    int nl, fixed, has_pt;     // lines, the fixed line, whether the graph has a point function (not VP)
     const void *kfn[5];        // the kernels (eval nodes, eval list AB 0, eval list AB 1, dd nodes, dd cells)
 };
 
@@ -205,7 +222,8 @@ template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 0>(SFX_EVAL_ARGS);      
 template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 1>(SFX_EVAL_ARGS);                                   \
 template __global__ void sfx::k_dd<G, false>(SFX_DD_ARGS);                                                   \
 template __global__ void sfx::k_dd<G, true>(SFX_DD_ARGS);                                                    \
-template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, int, double*, double*, double*, double*);
+template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, int, double*, double*, double*, double*); \
+template __global__ void sfx::k_pt<G>(long, const double*, const double*, double*, double*);
 #else
 #define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
 static hipError_t sfx_eval_##NAME(hipStream_t s, int mode, int ab, const sfx_cell_t *pts, const uint32_t *list, \
@@ -236,8 +254,15 @@ static hipError_t sfx_raw_##NAME(hipStream_t s, long n, const double *sf, int pr
                        (const sfx_cell_t*)nullptr, n, sf, prec, o1, o2, o3, o4);                             \
     return hipGetLastError();                                                                                \
 }                                                                                                            \
+static hipError_t sfx_pt_##NAME(hipStream_t s, long n, const double *xhi, const double *xlo, double *vhi,    \
+                                double *vlo){                                                                \
+    if(n <= 0) return hipSuccess;                                                                            \
+    hipLaunchKernelGGL((sfx::k_pt<G>), dim3((unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK)), dim3(SFX_BLOCK), 0, s, \
+                       n, xhi, xlo, vhi, vlo);                                                               \
+    return hipGetLastError();                                                                                \
+}                                                                                                            \
 extern "C" const sfx_graph_entry SFX_E_##NAME = {#NAME, VOLKOV, L, D, W,                                      \
-    sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME,                                                          \
+    sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME, sfx_pt_##NAME, G::NL, G::FIXED, G::HAS_PT ? 1 : 0,       \
     {(const void*)&sfx::k_eval<G, sfx::MODE_NODES, 2>, (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 0>,     \
      (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 1>,                                                        \
      (const void*)&sfx::k_dd<G, false>, (const void*)&sfx::k_dd<G, true>}};

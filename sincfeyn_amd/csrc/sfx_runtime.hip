@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <math.h>
 #include <vector>
@@ -435,6 +436,80 @@ extern "C" int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t
     std::vector<double> f(m), a(m);
     if(m && (rc = eval_weighted(graph, h, K, m, kk.data(), f.data(), a.data()))) return rc;
     for(long j = 0; j < m; j++){ fw[at[j]] = f[j]; abw[at[j]] = a[j]; }
+    return SFX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// scale test (reference main.c:79-116): the points on the host in double-double, the point function on
+// the device, the measure product and the ratios on the host, directions in index order
+extern "C" int sfx_scale_test(int graph, double kappa, unsigned seed, int E, sfx_scale_result_t *out){
+    int rc;
+    if((rc = require_init()) || (rc = check_graph(graph))) return rc;
+    if(!out || E < 1 || E > 8){ set_error("sfx_scale_test: bad arguments (E in 1..8)"); return SFX_EINVAL; }
+    const sfx_graph_entry *g = SFX_GRAPHS[graph];
+    if(!g->has_pt){ set_error("sfx_scale_test: %s has no point function (VP graph)", g->name); return SFX_EUNSUP; }
+    const int n = g->nl, fixed = g->fixed, B = 2*E + 1;
+    if(n < 2 || n > 16 || fixed < 0 || fixed >= n){ set_error("sfx_scale_test: %s: %d lines", g->name, n); return SFX_EINVAL; }
+    long ndir = 1;
+    for(int i = 0; i < n - 1; i++){ ndir *= B; if(ndir > (1L << 24)){ set_error("sfx_scale_test: (2E+1)^(n-1) too large"); return SFX_EINVAL; } }
+    memset(out, 0, sizeof *out);
+    out->n = n; out->fixed = fixed; out->ndir = ndir - 1; out->worstd = -1;
+    srand(seed);
+    sfx_dd x0[16];
+    for(int i = 0; i < n; i++) x0[i] = sfx_dd(exp(4.0*rand()/RAND_MAX - 2.0));
+    x0[fixed] = sfx_dd(1.0);
+    // every (direction, kappa) point: x (hi, lo) for the device, the measure kept on the host
+    std::vector<double> xh, xl; std::vector<sfx_dd> meas; std::vector<long> dir;
+    xh.reserve((size_t)ndir*3*n); xl.reserve((size_t)ndir*3*n);
+    for(long d = 0; d < ndir; d++){
+        int e[16], nz = 0; long r = d;
+        for(int i = 0; i < n; i++){ if(i == fixed){ e[i] = 0; continue; } e[i] = (int)(r % B) - E; r /= B; nz += e[i] != 0; }
+        if(!nz) continue;
+        dir.push_back(d);
+        for(int q = 0; q < 3; q++){
+            double kk = kappa*(q + 1)/3.0; sfx_dd m(1.0);
+            for(int i = 0; i < n; i++){
+                sfx_dd x = x0[i]*sfx_dd(exp(-kk*e[i]));
+                xh.push_back(x.hi); xl.push_back(x.lo);
+                if(i != fixed) m *= x;
+            }
+            meas.push_back(m);
+        }
+    }
+    const long np = (long)meas.size();
+    DevBuf dx, dv;
+    if((rc = dx.reserve((size_t)np*n*2*sizeof(double))) || (rc = dv.reserve((size_t)np*2*sizeof(double)))){ dx.release(); dv.release(); return rc; }
+    hipStream_t s = lib().stream;
+    double *dxh = dx.as<double>(), *dxl = dxh + (size_t)np*n, *dvh = dv.as<double>(), *dvl = dvh + np;
+    std::vector<double> vh(np), vl(np);
+    hipError_t e = hipMemcpyAsync(dxh, xh.data(), (size_t)np*n*sizeof(double), hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = hipMemcpyAsync(dxl, xl.data(), (size_t)np*n*sizeof(double), hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = g->pt(s, np, dxh, dxl, dvh, dvl);
+    if(e == hipSuccess) e = hipMemcpyAsync(vh.data(), dvh, np*sizeof(double), hipMemcpyDeviceToHost, s);
+    if(e == hipSuccess) e = hipMemcpyAsync(vl.data(), dvl, np*sizeof(double), hipMemcpyDeviceToHost, s);
+    if(e == hipSuccess) e = hipStreamSynchronize(s);
+    dx.release(); dv.release();
+    if(e != hipSuccess) return hip_fail(e, "sfx_scale_test", __FILE__, __LINE__);
+    for(size_t j = 0; j < dir.size(); j++){
+        long d = dir[j]; double gv[3];
+        for(int q = 0; q < 3; q++){ sfx_dd v(vh[3*j + q], vl[3*j + q]); gv[q] = fabs((double)(v*meas[3*j + q])); }
+        double ratio = gv[1] > 0 ? gv[2]/gv[1] : 0;
+        int ee[16]; long r = d;
+        for(int i = 0; i < n; i++){ if(i == fixed){ ee[i] = 0; continue; } ee[i] = (int)(r % B) - E; r /= B; }
+        if(ratio > out->worst){
+            out->worst = ratio; out->worstd = d;
+            for(int q = 0; q < 3; q++) out->g_worst[q] = gv[q];
+            for(int i = 0; i < n; i++) out->e_worst[i] = ee[i];
+        }
+        if(ratio > 0.3 && gv[2] > 1e-30){
+            if(out->nslow < SFX_SCALE_MAXSLOW){
+                for(int q = 0; q < 3; q++) out->g_slow[out->nslow][q] = gv[q];
+                for(int i = 0; i < n; i++) out->e_slow[out->nslow][i] = ee[i];
+                out->nslow++;
+            }
+            out->bad++;
+        }
+    }
     return SFX_OK;
 }
 

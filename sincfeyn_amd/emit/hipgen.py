@@ -32,7 +32,7 @@ import os
 import re
 import sys
 
-DESC_RE = re.compile(r'^const graph_t G_(\w+)=\{"([^"]*)","([^"]*)",(\d+),(\d+),(\d+),(\d+),(\d+),', re.M)
+DESC_RE = re.compile(r'^const graph_t G_(\w+)=\{"([^"]*)","([^"]*)",(\d+),(\d+),(\d+),(\d+),(\d+),([^}]*)\}', re.M)
 GL_RE = re.compile(r'^static const double (GL[A-Z]*X_\w+)\[(\d+)\]=\{([^}]*)\}, (GL[A-Z]*W_\w+)\[(\d+)\]=\{([^}]*)\};', re.M)
 DEF_RE = re.compile(r'^static (REAL|void) FN\((\w+)\)\(', re.M)
 
@@ -62,8 +62,12 @@ def parse_kernel(text):
         raise EmitError('no graph_t descriptor')
     if m.group(2) != m.group(1):
         raise EmitError('descriptor G_%s names graph %s' % (m.group(1), m.group(2)))
+    # the point function (ptq, common.h:38) is present for graphs with a single-point body; VP graphs
+    # (vacuum-polarisation insertions) have 0 there
+    fns = [t.strip() for t in m.group(9).split(',')]
     desc = dict(name=m.group(1), volkov=m.group(3), L=int(m.group(4)), D=int(m.group(5)),
-                weight=int(m.group(6)), n=int(m.group(7)), fixed=int(m.group(8)))
+                weight=int(m.group(6)), n=int(m.group(7)), fixed=int(m.group(8)),
+                has_pt=len(fns) >= 4 and fns[3] == 'pt_%s_q' % m.group(1))
     start = text.find('#define REAL QREAL')
     if start < 0:
         raise EmitError('no QREAL half')
@@ -179,6 +183,11 @@ def emit(src_path, out_dir):
              '    static constexpr int D = %d;' % desc['D'],
              '    template<class REAL> static __device__ __forceinline__ REAL f(const REAL *sf, REAL *ab, const int sqab)',
              '    { return f_%s<REAL>(sf, ab, sqab); }' % name,
+             '    static constexpr int NL = %d, FIXED = %d;' % (desc['n'], desc['fixed']),
+             '    static constexpr bool HAS_PT = %s;' % ('true' if desc['has_pt'] else 'false'),
+             '    template<class REAL> static __device__ __forceinline__ REAL pt(const REAL *x, REAL *ab)',
+             ('    { return pt_%s<REAL>(x, (REAL)0, ab, 1); }' % name) if desc['has_pt'] else
+             '    { (void)x; *ab = (REAL)0; return (REAL)0; }',
              '};',
              '}  // namespace sfxk_%s' % name,
              'SFX_GRAPH_ENTRY(%s, sfxk_%s::Graph, "%s", %d, %d, %d)' % (

@@ -42,6 +42,12 @@ class GraphInfo(C.Structure):
                 ('vgprs', C.c_int), ('sgprs', C.c_int), ('scratch_bytes', C.c_int), ('occupancy', C.c_int)]
 
 
+class ScaleResult(C.Structure):
+    _fields_ = [('n', C.c_int), ('fixed', C.c_int), ('ndir', C.c_long), ('bad', C.c_int), ('worst', C.c_double),
+                ('worstd', C.c_long), ('e_worst', C.c_int*16), ('g_worst', C.c_double*3), ('nslow', C.c_int),
+                ('e_slow', (C.c_int*16)*12), ('g_slow', (C.c_double*3)*12)]
+
+
 class ProfEntry(C.Structure):
     _fields_ = [('graph', C.c_char * 32), ('kind', C.c_int), ('launches', C.c_long), ('ms', C.c_double),
                 ('points', C.c_long), ('skipped', C.c_long), ('dd_points', C.c_long)]
@@ -173,6 +179,22 @@ def eval_cells(graph, h, K, cells, st, skip=None):
     _check(lib().sfx_eval_cells(graph, h, K, nc, cells.ctypes.data, None if sk is None else sk.ctypes.data, st,
                                 fw.ctypes.data, abw.ctypes.data))
     return fw, abw
+
+
+def scale_test(graph, kappa=24.0, seed=1, E=1):
+    """`sincqed scale` for one graph (main.c:79-116); None for a VP graph (no point function)"""
+    g = find(graph) if isinstance(graph, str) else graph
+    r = ScaleResult()
+    L = lib()
+    L.sfx_scale_test.argtypes = [C.c_int, C.c_double, C.c_uint, C.c_int, C.POINTER(ScaleResult)]
+    rc = L.sfx_scale_test(g, kappa, seed, E, C.byref(r))
+    if rc == -5:
+        return None
+    _check(rc)
+    n = r.n
+    return dict(n=n, fixed=r.fixed, ndir=r.ndir, bad=r.bad, worst=r.worst, worstd=r.worstd,
+                e_worst=list(r.e_worst)[:n], g_worst=list(r.g_worst),
+                slow=[(list(r.e_slow[k])[:n], list(r.g_slow[k])) for k in range(r.nslow)])
 
 
 def eval_finished(graph, h, K, k, abserr, skip=None, **kw):

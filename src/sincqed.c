@@ -1,7 +1,8 @@
-/* sincqed.c -- the command-line host of the MI355X evaluator: `sincqed list` and
- * `sincqed run h N [eps] [top] GRAPH...` with the reference's arguments, environment, log lines and
- * per-graph results line (reasther/sincFeyn driver/main.c:40-75, 156-251), every graph's hierarchical
- * sum computed on the GPU through the C ABI of include/sfx.h.
+/* sincqed.c -- the command-line host of the MI355X evaluator: `sincqed list`,
+ * `sincqed run h N [eps] [top] GRAPH...`, `sincqed scale GRAPH [kappa] [seed]` and
+ * `sincqed compare R1 R2`, with the reference's arguments, environment, log lines and output formats
+ * (reasther/sincFeyn driver/main.c:40-251), every graph's hierarchical sum and every integrand value
+ * computed on the GPU through the C ABI of include/sfx.h.
  *
  * As in the reference's GPU build, the symmetry fold is not applied (sum.c:571): SINCQED_SYM is ignored
  * with a note.  SINCQED_GPU=d picks the device (gpu.cu:30-34).
@@ -83,6 +84,76 @@ static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opt
     return 0;
 }
 
+/* ---- compare (main.c:118-153): the error of each graph from two results files at different h ---- */
+typedef struct { char name[32]; double h, v, rnd; } cres_t;
+/* a results line holds the Volkov string (with blanks) before the numbers, so the value column is found
+ * as the first signed %+.15e token; h is 4 columns before it and the round column 5 after (the format of
+ * run_graph below and main.c:66-73); a later line of the same graph replaces an earlier one */
+static int read_results(const char *fn, cres_t *r, int max){
+    FILE *f = fopen(fn, "r"); if(!f){ perror(fn); return -1; }
+    char line[4096]; int n = 0;
+    while(fgets(line, sizeof line, f)){
+        if(line[0] == '#') continue;
+        char *tok[64]; int nt = 0;
+        for(char *t = strtok(line, " \t\n"); t && nt < 64; t = strtok(NULL, " \t\n")) tok[nt++] = t;
+        int iv = -1;
+        for(int i = 1; i < nt && iv < 0; i++)
+            if((tok[i][0] == '+' || tok[i][0] == '-') && strlen(tok[i]) >= 20 && strchr(tok[i], 'e')) iv = i;
+        if(iv < 5 || iv + 5 >= nt) continue;
+        int j = 0; while(j < n && strcmp(r[j].name, tok[0])) j++;
+        if(j == n){ if(n == max) continue; n++; }
+        snprintf(r[j].name, sizeof r[j].name, "%s", tok[0]);
+        r[j].h = atof(tok[iv - 4]); r[j].v = atof(tok[iv]); r[j].rnd = atof(tok[iv + 5]);
+    }
+    fclose(f);
+    return n;
+}
+static int compare(const char *fa, const char *fb){
+    static cres_t A[2048], B[2048];
+    int na = read_results(fa, A, 2048), nb = read_results(fb, B, 2048);
+    if(na < 0 || nb < 0) return 1;
+    printf("# compare %s and %s: value from the smaller step h; error = |S(h) - S(h')|/3 + round\n", fa, fb);
+    printf("# (calibrated on graph 16, LbL and the Pi2 Pi2 chain with h = 1.0, h' = 1.2: true error <= |dS|/3)\n");
+    printf("# %-8s %6s %6s  %22s  %10s  %10s\n", "name", "h", "h'", "value", "dS", "error");
+    double tot = 0, e1 = 0, e2 = 0; int m = 0;
+    for(int i = 0; i < na; i++){
+        int j = 0; while(j < nb && strcmp(A[i].name, B[j].name)) j++;
+        if(j == nb) continue;
+        const cres_t *fine = A[i].h <= B[j].h ? &A[i] : &B[j], *coarse = A[i].h <= B[j].h ? &B[j] : &A[i];
+        if(fine->h == coarse->h){ printf("  %-8s same h in both files, skipped\n", fine->name); continue; }
+        double d = fine->v - coarse->v, e = fabs(d)/3 + fine->rnd;
+        printf("  %-8s %6.3f %6.3f  %+22.15e  %+10.2e  %10.2e\n", fine->name, fine->h, coarse->h, fine->v, d, e);
+        tot += fine->v; e1 += e; e2 += e*e; m++;
+    }
+    printf("# SUM %d graphs  %+.12f  err %.2e (in quadrature)  %.2e (linear)\n", m, tot, sqrt(e2), e1);
+    return 0;
+}
+
+/* ---- scale (main.c:77-116): decay of integrand x measure along every scaling direction ---- */
+static int scale(const char *sel, double kap, unsigned seed, int E){
+    int ng = sfx_graph_count(), rc = sfx_init(envi("SINCQED_GPU", 0));
+    if(rc){ fprintf(stderr, "sincqed: %s\n", sfx_last_error()); return 3; }
+    for(int i = 0; i < ng; i++){
+        if(!sfx_graph_selected(i, sel)) continue;
+        sfx_graph_info_t g; sfx_graph_info(i, &g);
+        static sfx_scale_result_t S;
+        rc = sfx_scale_test(i, kap, seed, E, &S);
+        if(rc == SFX_EUNSUP){ printf("%-6s %-22s (no scale test for VP graphs)\n", g.name, g.volkov); continue; }
+        if(rc){ fprintf(stderr, "sincqed: %s: %s\n", g.name, sfx_last_error()); return 4; }
+        for(int k = 0; k < S.nslow; k++){
+            printf("  slow direction e=(");
+            for(int j = 0; j < S.n; j++) if(j != S.fixed) printf("%+d", S.e_slow[k][j]);
+            printf(")  |g| at kappa/3,2/3,1: %.3e %.3e %.3e\n", S.g_slow[k][0], S.g_slow[k][1], S.g_slow[k][2]);
+        }
+        printf("%-6s %-22s scale test kappa=%g: %ld directions, %d slow (ratio>0.3), worst ratio %.3g (dir %ld: e=(",
+               g.name, g.volkov, kap, S.ndir, S.bad, S.worst, S.worstd);
+        for(int j = 0; j < S.n; j++) if(j != S.fixed) printf("%+d", S.e_worst[j]);
+        printf(") |g| %.2e %.2e %.2e)\n", S.g_worst[0], S.g_worst[1], S.g_worst[2]);
+    }
+    sfx_shutdown();
+    return 0;
+}
+
 int main(int argc, char **argv){
     sfx_opts_t o; sfx_opts_default(&o);
     o.abserr0 = envd("G2_ABSERR", o.abserr0); o.abrel = envd("SINCQED_ABREL", o.abrel);
@@ -94,10 +165,18 @@ int main(int argc, char **argv){
         printf("%d graphs (one per mirror pair; w = mirror weight), precision fallback double-double on the GPU\n", ng);
         for(int i = 0; i < ng; i++){ sfx_graph_info_t g; sfx_graph_info(i, &g);
             printf("  %-8s L=%d  %-22s D=%d w=%d\n", g.name, g.L, g.volkov, g.D, g.weight); }
-        printf("usage: %s run h N [eps] [top] GRAPH...\n", argv[0]);
+        printf("usage: %s run h N [eps] [top] GRAPH... | scale GRAPH [kappa] [seed] | compare R1 R2\n", argv[0]);
         return 0;
     }
-    if(strcmp(argv[1], "run")){ fprintf(stderr, "unknown command %s (this build: list, run)\n", argv[1]); return 1; }
+    if(!strcmp(argv[1], "scale")){
+        if(argc < 3) return 1;
+        return scale(argv[2], argc > 3 ? atof(argv[3]) : 24, argc > 4 ? (unsigned)atoi(argv[4]) : 1, envi("SCALE_E", 1));
+    }
+    if(!strcmp(argv[1], "compare")){
+        if(argc < 4){ fprintf(stderr, "compare RESULTS_h RESULTS_h2   (two results files, e.g. h = 1.0 and 1.2)\n"); return 1; }
+        return compare(argv[2], argv[3]);
+    }
+    if(strcmp(argv[1], "run")){ fprintf(stderr, "unknown command %s\n", argv[1]); return 1; }
     if(argc < 5){ fprintf(stderr, "run h N [eps] [top] GRAPH...\n"); return 1; }
     double h = atof(argv[2]); int N = atoi(argv[3]), a = 4;
     double eps = 1e-11; int top = -1;

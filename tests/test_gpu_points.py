@@ -92,3 +92,20 @@ def test_eval_cells_matches_nodes(sfx):
             else:
                 f1, a1 = sfx.eval_nodes(gi, h, K, kk[None, :])
                 assert fw[i] == f1[0] and abw[i] == a1[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,E', [('g2_00', 1), ('g2_01', 2), ('g2_03', 1), ('g3_00', 1), ('g3_07', 2),
+                                    ('g3_lbl', 1)])
+def test_scale_test_matches_oracle(sfx, oracle, name, E):
+    """`sincqed scale` (main.c:79-116): the device's double-double point function and the host's
+    direction loop give the oracle's result bit for bit -- counts, worst direction, every |g|; at E = 2 this
+    includes the directions whose |g| sits at the double-double floor (g3_07 has one 'slow' one there)"""
+    d = sfx.scale_test(name, 24.0, 1, E)
+    o = oracle.scale(name, 24.0, 1, E)
+    assert d == o, (name, E)
+
+
+@pytest.mark.gpu
+def test_scale_test_vp_graph(sfx):
+    assert sfx.scale_test('v3_00') is None

@@ -126,3 +126,26 @@ def test_dd_against_fp64_error_bound():
         assert np.all(np.abs(v - f) <= bound + 1e-300), g
         # the pair is normalised: |lo| <= ulp(hi)/2
         assert np.all(np.abs(dd[:, 1]) <= np.spacing(np.abs(dd[:, 0]))/2 + 1e-300), g
+
+
+@pytest.mark.parametrize('g', ['g2_00', 'g2_01', 'g2_02', 'g2_03', 'g3_00', 'g3_07', 'g3_21'])
+def test_scale_oracle_invariants(g):
+    """the restated scale test (main.c:79-116) on the reference's own point functions: (2E+1)^(n-1) - 1
+    directions; at E = 1 no slow direction for these subtracted integrands, and the slowest decay of |pt x measure|
+    between kappa 2/3 and kappa is the one power of rho that a single-line scaling leaves, e^(-kappa/3)
+    (to 1e-4 at kappa = 24), on a direction that moves one line only -- a wrong exponent sign, kappa step
+    or fixed line breaks one of these"""
+    o = O.Oracle(nc=True)
+    r = o.scale(g, 24.0, 1, 1)
+    assert r['ndir'] == 3**(r['n'] - 1) - 1
+    assert r['bad'] == 0 and r['slow'] == []
+    assert abs(r['worst']/math.exp(-8.0) - 1) < 1e-4
+    assert r['e_worst'][r['fixed']] == 0
+    assert r['g_worst'][0] > r['g_worst'][1] > r['g_worst'][2] > 0
+    # E = 2 reaches scalings where |g| sinks to the double-double floor (1e-16 .. 1e-23) and ratios are
+    # roundoff; only the direction count is fixed there
+    assert o.scale(g, 24.0, 1, 2)['ndir'] == 5**(r['n'] - 1) - 1
+
+
+def test_scale_oracle_vp_graph_has_none():
+    assert O.Oracle(nc=True).scale('v3_00') is None
