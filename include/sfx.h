@@ -33,6 +33,7 @@ extern "C" {
 #define SFX_ENOMEM     -3   /* device or host allocation failed */
 #define SFX_ENOTINIT   -4   /* sfx_init not called */
 #define SFX_EUNSUP     -5   /* a mode this build does not support */
+#define SFX_ESTOPPED   -6   /* stopped after a checkpoint (sfx_opts_t.ckpt_stop_after); resume from the file */
 
 typedef struct {
     char name[32];        /* "g3_00" */
@@ -55,6 +56,14 @@ typedef struct {
     int btmargin;         /* SINCQED_BTMARGIN, default 2 */
     long block_points;    /* children per refinement block on the device (0 = library default) */
     int verbose;          /* 1: progress lines on stderr */
+    /* checkpoints (checkpoint.c:1-6): the file <ckpt_dir>/sincqed_<graph>.ckpt in the reference's SQCK4
+     * format, written every ckpt_every seconds between refinement blocks and at the end of every level,
+     * replaced atomically and removed when the graph finishes; read back at the start of a sum whose h,
+     * N, eps and top match, so CPU (reference) and GPU runs resume each other */
+    double ckpt_every;    /* SINCQED_CKPT: seconds between checkpoints, 0 = off (the library's default) */
+    char ckpt_dir[256];   /* SINCQED_CKPT_DIR, "" = current directory */
+    int ckpt_stop_after;  /* stop the sum with SFX_ESTOPPED right after its n-th checkpoint (0 = never):
+                             a job evicted this way resumes from its file */
 } sfx_opts_t;
 
 typedef struct {
@@ -73,6 +82,10 @@ typedef struct {
     double abserr;        /* the fallback threshold used by the refinement */
     int K, top, levels;   /* lattice half-width after rounding, coarsest level, levels run */
     double t_grow, t_dilate, t_refine;   /* seconds */
+    int resumed;          /* the sum continued from a checkpoint: at level resume_lev, cell resume_next of */
+    int resume_lev;       /*   resume_ncells (sum.c:602) */
+    long resume_next, resume_ncells;
+    int ckpts_written;    /* checkpoints this call wrote */
 } sfx_result_t;
 
 /* ---- library ---- */

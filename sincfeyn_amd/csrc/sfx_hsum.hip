@@ -215,8 +215,15 @@ __global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, lon
 }
 __global__ void k_zero_u32(uint32_t *p, long n){ for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) p[i] = 0; }
 
-// one refinement level on the device: the children of cur[0..ncur) in blocks; returns the level's sums
-static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out){
+#include "sfx_ckpt.inc"
+
+// where a level starts: at cell c0 with the partial sums acc and the kept children nxt of the cells
+// before it (a resumed checkpoint), or at 0 with nothing
+struct LevelStart { long c0 = 0; const LevelAcc *acc = nullptr; const std::vector<sfx_cell_t> *nxt = nullptr; };
+
+// one refinement level on the device: the children of cur[c0..ncur) in blocks; returns the level's sums
+// (a checkpoint between blocks when one is due: ck)
+static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out, const LevelStart &ls, CkRun &ck){
     JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; hipStream_t s = J.s; int rc;
     const int D = J.D, st = 1 << lev; const bool last = lev == 0;
     const double pw = pow((double)st, (double)D), thr = J.eps*ref*1e-2;
@@ -231,20 +238,25 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     // the next level's list at its bound (every child kept; the job's HBM reservation covers it), so the
     // blocks run back to back without a host round trip; per block three counters (two class lists and
     // the double-double list), all zeroed here
-    const long nblk = (ncur + Bc - 1)/Bc;
-    if(!last && (rc = B.nxt.reserve((size_t)(ncur << D)*sizeof(sfx_cell_t)))) return rc;
-    if((rc = B.lcnt.reserve((size_t)nblk*4*sizeof(uint32_t)))) return rc;
-    hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<long>((nblk*4 + 255)/256, 1024)), dim3(256), 0, s, B.lcnt.as<uint32_t>(), nblk*4);
+    const long nblk = (ncur - ls.c0 + Bc - 1)/Bc, nk0 = ls.nxt ? (long)ls.nxt->size() : 0;
+    if(!last && (rc = B.nxt.reserve((size_t)(nk0 + ((ncur - ls.c0) << D))*sizeof(sfx_cell_t)))) return rc;
+    if((rc = B.lcnt.reserve((size_t)std::max<long>(nblk, 1)*4*sizeof(uint32_t)))) return rc;
+    if(ls.acc){
+        SFX_HIP(hipMemcpyAsync(B.acc.p, ls.acc, sizeof(LevelAcc), hipMemcpyHostToDevice, s));
+        if(!last && nk0) SFX_HIP(hipMemcpyAsync(B.nxt.p, ls.nxt->data(), nk0*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
+        SFX_HIP(hipStreamSynchronize(s));
+    } else hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(1), 0, s, B.acc.as<LevelAcc>());
+    if(nblk) hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<long>((nblk*4 + 255)/256, 1024)), dim3(256), 0, s, B.lcnt.as<uint32_t>(), nblk*4);
     long prof_nv = 0, prof_nsk = 0, prof_nq = 0;
-    for(long c0 = 0; c0 < ncur; c0 += Bc){
+    if(ls.acc){ prof_nv = (long)ls.acc->nv; prof_nsk = (long)ls.acc->nskip; prof_nq = (long)ls.acc->nq; }
+    for(long c0 = ls.c0; c0 < ncur; c0 += Bc){
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
         const bool pon = prof().on;
         if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
         // the block's cells in two lists by class, each run by the integrand compiled for its class; the
         // evaluations append the slots that need the double-double recompute to ddidx
-        uint32_t *lc = B.lcnt.as<uint32_t>() + 4*(c0/Bc);
+        uint32_t *lc = B.lcnt.as<uint32_t>() + 4*((c0 - ls.c0)/Bc);
         hipLaunchKernelGGL(k_cell_split, dim3((unsigned)((nc + 256*SPLIT_ITEMS - 1)/(256*SPLIT_ITEMS))), dim3(256), 0, s, cells, nc, J.o.abskip,
                            B.lsk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc);
         if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
@@ -275,6 +287,28 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
             prof_add(J.g->name, SFX_K_LEVEL_SUM, prof_ms(X, 4, 5), n, 0, 0);
         }
         if(J.o.verbose) fprintf(stderr, "  level %d: cells %ld/%ld\n", lev, c0 + nc, ncur);
+        if(ck.ck->due() && c0 + nc < ncur){
+            // a checkpoint after this block (sum.c:782-783): the device is drained first
+            LevelAcc a_; SFX_HIP(hipMemcpyAsync(&a_, B.acc.p, sizeof a_, hipMemcpyDeviceToHost, s));
+            SFX_HIP(hipStreamSynchronize(s));
+            if(ck.cur_lev != lev){ if((rc = download_cells(B.cur, ncur, s, ck.cur))) return rc; ck.cur_lev = lev; }
+            std::vector<sfx_cell_t> kept;
+            if(!last && (rc = download_cells(B.nxt, (long)a_.nkept, s, kept))) return rc;
+            CkState c;
+            c.lev = lev; c.next = c0 + nc;
+            c.s = a_.s[0] + a_.s[1]; c.a = a_.a[0] + a_.a[1]; c.drop = a_.drop[0] + a_.drop[1]; c.u = a_.u[0] + a_.u[1];
+            c.ne = (long)a_.nv; c.q = (long)a_.nq;
+            ck_fill_run(c, ck.res, ck.Slev, ck.nl, now() - ck.t0);
+            cells_to_pts(ck.cur, D, c.cur); cells_to_pts(kept, D, c.nxt);
+            c.es[0] = a_.b1[0] + a_.b1[1]; c.es[1] = a_.b2[0] + a_.b2[1];
+            c.cnt[1] = ck.res->abskipped + (long)a_.nskip;
+            ck_write(*ck.ck, c);
+            ck.res->ckpts_written = ck.ck->written;
+            if(ck.ck->stop_after && ck.ck->written >= ck.ck->stop_after){
+                set_error("%s: stopped after checkpoint %d (%s)", J.g->name, ck.ck->written, ck.ck->path.c_str());
+                return SFX_ESTOPPED;
+            }
+        }
     }
     SFX_HIP(hipMemcpyAsync(out, B.acc.p, sizeof *out, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
@@ -365,32 +399,67 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     if((rc = B.EX.reserve(EX.size()*sizeof(double)))) return rc;
     SFX_HIP(hipMemcpyAsync(B.EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
     set_params(J, J.o.abserr0);
-    int zero = 0;
-    if((rc = box_zero(J, top, &zero, &res->nev, &res->unresdrop))) return rc;
-    if(zero){ res->zero = 1; res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
-    GrowOut G;
-    long ncur = 0;                      // the grow and the dilation leave the sorted top-level cells in B.cur
-    if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc;
-    res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
-    res->nev_level[top] = G.nev;
-    res->ref = G.A; res->prune = G.drop;
-    double t1 = now(); res->t_grow = t1 - t0;
-    if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
-    { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
-    double Slev[16]; int nl = 0; Slev[nl++] = G.S;
-    res->kept_top = ncur;
-    double t2 = now(); res->t_dilate = 0;
-    double abserr = fmax(J.o.abserr0, J.o.abrel*eps*G.A);          // sum.c:651
+    Ckpt ck; ck_setup(ck, J, N);
+    CkState C;
+    const bool resumed = ck.on && ck_read(ck, C) == 1;
+    double Slev[16]; int nl = 0;
+    long ncur = 0;                      // the top-level cells (sorted, from the grow and the dilation) in B.cur
+    int lev0 = top - 1;
+    LevelStart ls0; LevelAcc acc0; std::vector<sfx_cell_t> nxt0;
+    double ref = 0, t2;
+    if(resumed){
+        // sum.c:596-603: the run continues at level C.lev, cell C.next, with the file's sums and lists
+        t0 -= C.R.t;
+        res->resumed = 1; res->resume_lev = C.lev; res->resume_next = C.next; res->resume_ncells = (long)C.cur.size();
+        res->ref = ref = C.R.ref; res->prune = C.R.prune; res->trunc = C.R.trunc; res->unres = C.R.unres;
+        res->nev = C.R.nev; res->nq = C.R.nq; res->kept_top = C.R.kept_top; res->abskipped = C.cnt[1];
+        memcpy(Slev, C.Slev, sizeof(double)*C.nl); nl = C.nl;
+        std::vector<sfx_cell_t> cur; pts_to_cells(C.cur, J.D, cur);
+        ncur = (long)cur.size();
+        if((rc = B.cur.reserve(std::max<long>(ncur, 1)*sizeof(sfx_cell_t)))) return rc;
+        if(ncur) SFX_HIP(hipMemcpyAsync(B.cur.p, cur.data(), ncur*sizeof(sfx_cell_t), hipMemcpyHostToDevice, J.s));
+        SFX_HIP(hipStreamSynchronize(J.s));
+        lev0 = C.lev;
+        if(C.next > 0){
+            memset(&acc0, 0, sizeof acc0);
+            acc0.s[0] = C.s; acc0.a[0] = C.a; acc0.drop[0] = C.drop; acc0.u[0] = C.u;
+            acc0.b1[0] = C.es[0]; acc0.b2[0] = C.es[1];
+            acc0.nv = (unsigned long long)C.ne; acc0.nq = (unsigned long long)C.q; acc0.nkept = C.nxt.size();
+            pts_to_cells(C.nxt, J.D, nxt0);
+            ls0.c0 = C.next; ls0.acc = &acc0; ls0.nxt = &nxt0;
+        }
+        t2 = now();
+    } else {
+        int zero = 0;
+        if((rc = box_zero(J, top, &zero, &res->nev, &res->unresdrop))) return rc;
+        if(zero){ res->zero = 1; res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
+        GrowOut G;
+        if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc;
+        res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
+        res->nev_level[top] = G.nev;
+        res->ref = ref = G.A; res->prune = G.drop;
+        double t1 = now(); res->t_grow = t1 - t0;
+        if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
+        { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
+        Slev[nl++] = G.S;
+        res->kept_top = ncur;
+        t2 = now(); res->t_dilate = 0;
+    }
+    double abserr = fmax(J.o.abserr0, J.o.abrel*eps*ref);          // sum.c:651
     res->abserr = abserr;
     set_params(J, abserr);
-    res->esok = J.D < 12;
+    res->esok = resumed ? C.esok : J.D < 12;
+    CkRun ckr; ckr.ck = &ck; ckr.res = res; ckr.Slev = Slev; ckr.t0 = t0;
     double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
     budget_acquire(held, 0);
-    for(int lev = top - 1; lev >= 0; lev--){
+    for(int lev = lev0; lev >= 0; lev--){
         long nnext = 0; LevelAcc A;
-        double want = lev > 0 ? (double)ncur*(double)(1L << J.D)*sizeof(sfx_cell_t) : 0;   // bound of the next list
+        const LevelStart none, &ls = lev == lev0 ? ls0 : none;
+        // the bound of the next list: every child of the cells still to do kept, plus those kept already
+        double want = lev > 0 ? ((double)(ncur - ls.c0)*(double)(1L << J.D) + (ls.nxt ? (double)ls.nxt->size() : 0))*sizeof(sfx_cell_t) : 0;
         budget_acquire(want, held);
-        if((rc = refine_level(J, lev, ncur, &nnext, G.A, &A))){ budget_release(held + want); return rc; }
+        ckr.nl = nl;
+        if((rc = refine_level(J, lev, ncur, &nnext, ref, &A, ls, ckr))){ budget_release(held + want); return rc; }
         if(lev > 0){ double now_ = (double)nnext*sizeof(sfx_cell_t); budget_release(held + want - now_); held = now_; }
         double S = A.s[0] + A.s[1];
         res->prune += A.drop[0] + A.drop[1];
@@ -402,9 +471,28 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
             res->unres = sqrt(A.u[0] + A.u[1]); res->Ah = A.a[0] + A.a[1];
             res->es1 = A.b1[0] + A.b1[1]; res->es2 = A.b2[0] + A.b2[1];
         }
-        if(lev > 0){ std::swap(B.cur, B.nxt); ncur = nnext; }
+        if(lev > 0){
+            std::swap(B.cur, B.nxt); ncur = nnext;
+            if(ck.on){
+                // the level-end checkpoint (sum.c:798): the next level's cells, nothing of it done yet
+                CkState c; c.lev = lev - 1; c.next = 0;
+                ck_fill_run(c, res, Slev, nl, now() - t0);
+                if((rc = download_cells(B.cur, ncur, J.s, ckr.cur))) { budget_release(held); return rc; }
+                ckr.cur_lev = lev - 1;
+                cells_to_pts(ckr.cur, J.D, c.cur);
+                c.cnt[1] = res->abskipped;
+                ck_write(ck, c);
+                res->ckpts_written = ck.written;
+                if(ck.stop_after && ck.written >= ck.stop_after){
+                    budget_release(held);
+                    set_error("%s: stopped after checkpoint %d (%s)", J.g->name, ck.written, ck.path.c_str());
+                    return SFX_ESTOPPED;
+                }
+            }
+        }
     }
     budget_release(held);
+    ck_remove(ck);
     res->levels = nl;
     res->S[0] = Slev[nl - 1]; res->S[1] = nl > 1 ? Slev[nl - 2] : NAN; res->S[2] = nl > 2 ? Slev[nl - 3] : NAN;
     res->t_refine = now() - t2;

@@ -22,10 +22,15 @@ class SfxError(RuntimeError):
     pass
 
 
+class SfxStopped(SfxError):
+    """the sum stopped after a checkpoint (opts ckpt_stop_after): run it again to resume from the file"""
+
+
 class Opts(C.Structure):
     _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
-                ('block_points', C.c_long), ('verbose', C.c_int)]
+                ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
+                ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int)]
 
 
 class Result(C.Structure):
@@ -34,7 +39,9 @@ class Result(C.Structure):
                 ('kept_top', C.c_long), ('es1', C.c_double), ('es2', C.c_double), ('esok', C.c_int), ('zero', C.c_int),
                 ('abskipped', C.c_long), ('unresdrop', C.c_long), ('btuns', C.c_long), ('nev_level', C.c_long * 16),
                 ('abserr', C.c_double), ('K', C.c_int), ('top', C.c_int), ('levels', C.c_int),
-                ('t_grow', C.c_double), ('t_dilate', C.c_double), ('t_refine', C.c_double)]
+                ('t_grow', C.c_double), ('t_dilate', C.c_double), ('t_refine', C.c_double),
+                ('resumed', C.c_int), ('resume_lev', C.c_int), ('resume_next', C.c_long), ('resume_ncells', C.c_long),
+                ('ckpts_written', C.c_int)]
 
 
 class GraphInfo(C.Structure):
@@ -85,7 +92,8 @@ def lib():
 
 def _check(rc):
     if rc != 0:
-        raise SfxError('sfx error %d: %s' % (rc, lib().sfx_last_error().decode()))
+        cls = SfxStopped if rc == -6 else SfxError
+        raise cls('sfx error %d: %s' % (rc, lib().sfx_last_error().decode()))
 
 
 _initialised = False
@@ -105,7 +113,7 @@ def opts(**kw):
     for k, v in kw.items():
         if not hasattr(o, k):
             raise KeyError(k)
-        setattr(o, k, v)
+        setattr(o, k, v.encode() if isinstance(v, str) else v)
     return o
 
 

@@ -49,6 +49,9 @@ static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opt
     sfx_result_t R;
     int rc = sfx_hsum(gi, h, N, eps, top, o, &R);
     if(rc){ fprintf(stderr, "sincqed: %s: %s\n", g.name, sfx_last_error()); return rc; }
+    if(R.resumed)                                          /* sum.c:602 */
+        say("# resuming %s from checkpoint: refine to step %dh, cell %ld of %ld\n", g.name, 1 << R.resume_lev,
+            R.resume_next, R.resume_ncells);
     if(R.zero) say("# %s: integrand is exactly zero at %d probe nodes; reported as zero without a scan\n", g.name, 4097);
     double w = g.weight, v = w*R.S[0], d2 = fabs(R.S[1] - R.S[0]);
     double disc = R.Ah > 0 ? 2*d2*d2/R.Ah : 0;
@@ -160,6 +163,8 @@ int main(int argc, char **argv){
     o.abskip = envi("SINCQED_ABSKIP", o.abskip); o.nearf = envd("SINCQED_NEARF", o.nearf);
     o.ddslack = envd("SINCQED_DDSLACK", o.ddslack); o.dropunres = envi("SINCQED_DROPUNRES", o.dropunres);
     o.keepbt = getenv("SINCQED_KEEPBT") != NULL; o.btmargin = envi("SINCQED_BTMARGIN", o.btmargin);
+    o.ckpt_every = envd("SINCQED_CKPT", 600);              /* checkpoint.c:9, main.c:184 */
+    { const char *d = getenv("SINCQED_CKPT_DIR"); if(d) snprintf(o.ckpt_dir, sizeof o.ckpt_dir, "%s", d); }
     int ng = sfx_graph_count();
     if(argc < 2 || !strcmp(argv[1], "list")){
         printf("%d graphs (one per mirror pair; w = mirror weight), precision fallback double-double on the GPU\n", ng);
