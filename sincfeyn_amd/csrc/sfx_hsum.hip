@@ -33,10 +33,16 @@ struct JobBufs {
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
     DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
            dkey, dns, dtr, dflag, skey, skey2, sortt;
-    void release(){ DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
-                                     &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
-                                     &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &dflag, &skey, &skey2, &sortt};
-                    for(DevBuf *b : all) b->release(); }
+    template<class F> void each(F f){
+        DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
+                         &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
+                         &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &dflag, &skey, &skey2, &sortt};
+        for(DevBuf *b : all) f(*b);
+    }
+    void release(){ each([](DevBuf &b){ b.release(); }); }
+    // free the buffers above `keep` bytes (a large job's cell lists and coarsest-level tables), so what a
+    // worker holds between jobs stays small and the engine's HBM budget describes the device
+    void trim(size_t keep){ each([keep](DevBuf &b){ if(b.cap > keep) b.release(); }); }
 };
 // a context: one HIP stream and its device buffers; the library's own context serves sfx_hsum and the
 // per-point calls, each engine worker has its own (so jobs on different workers overlap on the device)
@@ -574,6 +580,8 @@ static void worker_main(int w){
         }
         sfx_result_t r;
         int rc = hsum_core(X, j->graph, j->h, j->N, j->eps, j->top, &j->o, &r);
+        (void)hipStreamSynchronize(X.s);
+        X.B.trim((size_t)256 << 20);
         {
             std::lock_guard<std::mutex> lk(E.mu);
             j->res = r; j->rc = rc; j->t_end = now();
