@@ -40,9 +40,10 @@ def test_engine_jobs_match_oracle(sfx, oracle):
 
 
 @pytest.mark.gpu
-def test_serving_stream(sfx):
+def test_serving_stream(sfx, oracle):
     """a seeded Poisson stream (rate 40/s) of cheap jobs: every job served, each result identical to the
-    same job run alone"""
+    same job run alone, and each two-loop job's counts and value those of the CPU reference (the oracle,
+    no contraction)"""
     from sincfeyn_amd import serve
     sfx.engine_start(8, 0)
     graphs = sfx.select('all2', 'vp3')
@@ -51,8 +52,16 @@ def test_serving_stream(sfx):
     rep = serve.serve(stream, res)
     print(rep, flush=True)
     assert rep['jobs'] == len(stream) and rep['latency_p99_s'] >= rep['latency_p50_s'] > 0
+    names = {i: g['name'] for i, g in enumerate(sfx.graphs())}
+    n2 = 0
     for (a, g, h, N, eps, top), r in zip(stream, res):
         assert _same(r, sfx.hsum(g, h, N, eps, top)), (g, h, eps)
+        if names[g].startswith(('g2_', 'v2_')):
+            o = oracle.hsum(names[g], h, N, eps, top)
+            assert (r['nev'], r['nq'], r['kept_top']) == (o['nev'], o['nq'], o['kept_top']), names[g]
+            assert abs(r['S'][0] - o['S'][0]) <= 1e-12*abs(o['S'][0]), names[g]
+            n2 += 1
+    assert n2 > 0
     sfx.engine_stop()
 
 
