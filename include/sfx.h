@@ -64,6 +64,9 @@ typedef struct {
     char ckpt_dir[256];   /* SINCQED_CKPT_DIR, "" = current directory */
     int ckpt_stop_after;  /* stop the sum with SFX_ESTOPPED right after its n-th checkpoint (0 = never):
                              a job evicted this way resumes from its file */
+    int qd_fallback;      /* 1: the fallback recompute in quad-double instead of double-double (graphs with
+                             a quad-double kernel: the two-loop ones; else SFX_EUNSUP).  With abserr0 = 0,
+                             abrel = 0, abskip = 0 this is the all-QD error reference of a two-loop sum. */
 } sfx_opts_t;
 
 typedef struct {
@@ -123,6 +126,10 @@ int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t *cells, co
  * |g| = |pt(x) prod_{i != fixed} x_i| is taken in double-double (the point function on the device).
  * A direction is slow when |g(kappa)| / |g(2 kappa/3)| > 0.3 and |g(kappa)| > 1e-30.  SFX_EUNSUP for
  * the VP graphs, which have no point function. */
+/* Quad-double arithmetic of the all-QD mode, element-wise on n values of four doubles each (a[4i..4i+3]):
+ * op 0 a + b, 1 a - b, 2 a * b, 3 a / b; device 0 on the host (no GPU needed), 1 on the device. */
+int sfx_qd_ops(int op, long n, const double *a, const double *b, double *out, int device);
+
 #define SFX_SCALE_MAXSLOW 12
 typedef struct {
     int n, fixed;                        /* lines of the graph, the fixed line */
@@ -135,7 +142,8 @@ typedef struct {
 } sfx_scale_result_t;
 int sfx_scale_test(int graph, double kappa, unsigned seed, int E, sfx_scale_result_t *out);
 /* the integrand at given Sinc variables sf[i*D + d], no weight and no tests.  prec 0: f and Sum|terms|
- * in FP64 into out[2i], out[2i+1]; prec 1: double-double, out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
+ * in FP64 into out[2i], out[2i+1]; prec 1: double-double, out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo;
+ * prec 2: quad-double (two-loop graphs, else SFX_EUNSUP), each value split into two doubles as prec 1 */
 int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out);
 /* the per-point values with the reference's tests and double-double fallback (eval.c:33-61) as the
  * refinement applies them: v[i] (0 if not finite), flags[i] = SFX_PT_* bits; abserr is the fallback

@@ -44,6 +44,8 @@ int  orc_graph(const char *name, orc_graph_t *out);
 int  orc_eval_fd(const char *name, long n, const double *sf, double *f, double *ab);
 /* fq (double-double half) -> out[4i..4i+3] = f.hi, f.lo, ab.hi, ab.lo */
 int  orc_eval_fq(const char *name, long n, const double *sf, double *out);
+/* fq of the __float128 build (libsqk_f128.so, prefix "F_"), each value split into two doubles */
+int  orc_eval_fq128(const char *name, long n, const double *sf, double *out);
 int  orc_hsum(const char *name, double h, int N, double eps, int top, const orc_opts_t *o, orc_res_t *r);
 /* the scale test, driver/main.c:79-116 (directions in index order, one thread); -2 for a graph without
  * a point function (VP) */

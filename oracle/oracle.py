@@ -15,6 +15,7 @@ REF_DIR = os.path.join(HERE, '_ref')
 LIB_ORACLE = os.path.join(HERE, '_build', 'liboracle.so')
 LIB_KERNELS = os.path.join(REF_DIR, 'libsqk_ref.so')
 LIB_KERNELS_NC = os.path.join(REF_DIR, 'libsqk_nc.so')
+LIB_KERNELS_F128 = os.path.join(REF_DIR, 'libsqk_f128.so')
 SINCQED_REF = os.path.join(REF_DIR, 'sincqed_ref')
 SINCQED_REF_NC = os.path.join(REF_DIR, 'sincqed_ref_nc')
 
@@ -68,6 +69,7 @@ class Oracle:
         L.orc_graph.argtypes = [C.c_char_p, C.POINTER(OrcGraph)]
         L.orc_eval_fd.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_eval_fq.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p]
+        L.orc_eval_fq128.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.c_void_p]
         L.orc_hsum.argtypes = [C.c_char_p, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(OrcOpts), C.POINTER(OrcRes)]
         L.orc_opts_default.argtypes = [C.POINTER(OrcOpts)]
         L.orc_scale.argtypes = [C.c_char_p, C.c_double, C.c_uint, C.c_int, C.POINTER(OrcScale)]
@@ -110,6 +112,20 @@ class Oracle:
         if self.lib.orc_eval_fq(name.encode(), sf.shape[0], sf.ctypes.data, out.ctypes.data):
             raise RuntimeError(self.lib.orc_error().decode())
         return out
+
+    def eval_fq128(self, name, sf):
+        """the two-loop kernels built with QREAL = __float128: (n, 4) = f hi, lo, Sum|terms| hi, lo"""
+        import numpy as np
+        if self.lib.orc_open(LIB_KERNELS_F128.encode(), b'F_') != 0:
+            raise RuntimeError(self.lib.orc_error().decode())
+        try:
+            sf = np.ascontiguousarray(sf, dtype=np.float64)
+            out = np.zeros((sf.shape[0], 4))
+            if self.lib.orc_eval_fq128(name.encode(), sf.shape[0], sf.ctypes.data, out.ctypes.data):
+                raise RuntimeError(self.lib.orc_error().decode())
+            return out
+        finally:
+            self._select()
 
     def hsum(self, name, h, N, eps, top, **kw):
         self._select()

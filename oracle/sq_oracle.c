@@ -63,6 +63,22 @@ int orc_eval_fq(const char *name, long n, const double *sf, double *out){
     return 0;
 }
 
+/* fq of a kernel built with QREAL = __float128 (libsqk_f128.so, prefix "F_"): out[4i..4i+3] = f and
+ * Sum|terms| each split into two doubles (hi = the rounded value, lo = the rounded rest) */
+typedef __float128 (*fq128_t)(const __float128*, __float128*);
+int orc_eval_fq128(const char *name, long n, const double *sf, double *out){
+    const graph_t *g = findg(name); if(!g) return -1;
+    __float128 q[MAXD], ab;
+    for(long i = 0; i < n; i++){
+        for(int d = 0; d < g->D; d++) q[d] = sf[i*g->D + d];
+        __float128 v = ((fq128_t)g->fq)(q, &ab);
+        double vh = (double)v, ah = (double)ab;
+        out[4*i] = vh; out[4*i + 1] = (double)(v - (__float128)vh);
+        out[4*i + 2] = ah; out[4*i + 3] = (double)(ab - (__float128)ah);
+    }
+    return 0;
+}
+
 /* ---- the scale test (main.c:79-116) ---- */
 /* dd.h:18-20,38-40: the product of two double-double values (fma two-product, then renormalised) */
 static ddv dmul(ddv a, ddv b){

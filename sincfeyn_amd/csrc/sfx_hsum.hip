@@ -109,7 +109,7 @@ static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, doubl
     SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), nullptr, nullptr));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
-    SFX_HIP(J.g->dd(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
+    SFX_HIP((J.o.qd_fallback ? J.g->qd : J.g->dd)(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
                     B.v.as<double>(), B.e2.as<double>(), B.fl.as<uint8_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
     if(st > 0) hipLaunchKernelGGL(k_grow_shell, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, dpts, n, J.K, st, J.D, B.fl.as<uint8_t>());
@@ -272,7 +272,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         SFX_HIP(J.g->eval(s, MODE_LIST, 1, cells, B.lfu.as<uint32_t>(), lc + 1, n, st, &J.P, B.v.as<double>(),
                           B.fl.as<uint8_t>(), B.ddidx.as<uint32_t>(), lc + 2));
         if(pon) SFX_HIP(hipEventRecord(X.ev[3], s));
-        SFX_HIP(J.g->dd(s, 1, cells, st, B.ddidx.as<uint32_t>(), lc + 2, n, &J.P, B.v.as<double>(),
+        SFX_HIP((J.o.qd_fallback ? J.g->qd : J.g->dd)(s, 1, cells, st, B.ddidx.as<uint32_t>(), lc + 2, n, &J.P, B.v.as<double>(),
                         B.e2.as<double>(), B.fl.as<uint8_t>()));
         if(pon) SFX_HIP(hipEventRecord(X.ev[4], s));
         LevelArgs a;
@@ -356,6 +356,7 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.K = K; J.eps = 0; J.top = 0;
     J.ctx = &default_ctx(); J.s = J.ctx->s;
     if(opts) J.o = *opts; else sfx_opts_default(&J.o);
+    if(J.o.qd_fallback && !J.g->has_qd){ set_error("%s: no quad-double kernel (two-loop graphs only)", J.g->name); return SFX_EUNSUP; }
     std::vector<sfx_cell_t> pts(n);
     for(long i = 0; i < n; i++){
         memset(&pts[i], 0, sizeof pts[i]);
@@ -394,6 +395,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     int rc;
     Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.eps = eps; J.top = top; J.ctx = &X; J.s = X.s;
     if(opts) J.o = *opts; else sfx_opts_default(&J.o);
+    if(J.o.qd_fallback && !J.g->has_qd){ set_error("%s: no quad-double kernel (two-loop graphs only)", J.g->name); return SFX_EUNSUP; }
     int m = 1 << (top + 1); J.K = (N - 1)/2; J.K = ((J.K + m - 1)/m)*m;      // lattice.c:16
     memset(res, 0, sizeof *res);
     res->K = J.K; res->top = top; res->S[1] = res->S[2] = NAN;

@@ -19,6 +19,7 @@
 #include <float.h>
 #include <stdint.h>
 #include "sfx_dd.hpp"
+#include "sfx_qd.hpp"
 #include "sfx_abi_types.h"
 
 #define SFX_FN __device__ __forceinline__
@@ -126,13 +127,18 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
     }
 }
 
-// the double-double recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
+// the high-precision recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
 // the count); writes, at the point's own index i, the value, the eps^2 contribution to the round column
-// and the "unresolved" flag bit
-template<class G, bool CELLS>
-__global__ void __launch_bounds__(SFX_BLOCK)
-k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ cnt,
-     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ flout){
+// and the "unresolved" flag bit (qvalue, eval.c:25-30).  Q = sfx_dd (the reference's fallback) or
+// sfx_qd (the all-QD error reference of the two-loop graphs).
+template<class Q> struct hp_eps;
+template<> struct hp_eps<sfx_dd> { static constexpr double v = SFX_QEPS; };
+template<> struct hp_eps<sfx_qd> { static constexpr double v = SFX_QD_EPS; };
+
+template<class G, bool CELLS, class Q>
+__device__ __forceinline__ void hp_recompute(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx,
+                                             const uint32_t *__restrict__ cnt, const sfx_eval_params_t &P,
+                                             double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ flout){
     constexpr int D = G::D;
     long m = *cnt;
     for(long j = blockIdx.x*(long)blockDim.x + threadIdx.x; j < m; j += (long)gridDim.x*blockDim.x){
@@ -140,13 +146,13 @@ k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ id
         int k[D]; unsigned cfl;
         point_coords<D, CELLS>(pts, i, st, P.K, k, &cfl);
         double sf[D]; double w = lattice_node<D>(P.EX, P.K, P.H, k, sf);
-        sfx_dd sq[D];
+        Q sq[D];
 #pragma unroll
-        for(int d = 0; d < D; d++) sq[d] = sfx_dd(sf[d]);
-        sfx_dd abq;
-        sfx_dd vq = G::template f<sfx_dd>(sq, &abq, 1);
+        for(int d = 0; d < D; d++) sq[d] = Q(sf[d]);
+        Q abq;
+        Q vq = G::template f<Q>(sq, &abq, 1);
         double vqd = (double)vq, abqd = (double)abq;
-        double e = abqd*w*SFX_QEPS;                                   // eval.c:26
+        double e = abqd*w*hp_eps<Q>::v;                               // eval.c:26
         double e2 = (e*16 <= P.abserr) ? 0.0 : (isfinite(e) ? e*e : 0.0);
         double v = isfinite(vqd) ? vqd*w : 0;
         if(!(e <= fabs(v))){ flout[i] |= (uint8_t)SFX_FL_UNRES; if(P.dropunres) v = 0; }
@@ -154,8 +160,24 @@ k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ id
     }
 }
 
+template<class G, bool CELLS>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_dd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ cnt,
+     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ flout){
+    hp_recompute<G, CELLS, sfx_dd>(pts, st, idx, cnt, P, vout, e2out, flout);
+}
+
+// the same in quad-double; compiled only for the graphs that have it (G::QD: the two-loop integrands)
+template<class G, bool CELLS>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_qd(const sfx_cell_t *__restrict__ pts, int st, const uint32_t *__restrict__ idx, const uint32_t *__restrict__ cnt,
+     sfx_eval_params_t P, double *__restrict__ vout, double *__restrict__ e2out, uint8_t *__restrict__ flout){
+    if constexpr(G::QD) hp_recompute<G, CELLS, sfx_qd>(pts, st, idx, cnt, P, vout, e2out, flout);
+}
+
 // golden primitive: the integrand at nodes without the tests.  prec 0: o1 = f, o2 = Sum|terms| in FP64
-// (SQ_AB = 1); prec 1: o1 + o2 = f and o3 + o4 = Sum|terms| in double-double
+// (SQ_AB = 1); prec 1: o1 + o2 = f and o3 + o4 = Sum|terms| in double-double; prec 2 (graphs with G::QD):
+// the same two-double split of the quad-double values
 template<class G>
 __global__ void __launch_bounds__(SFX_BLOCK)
 k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfin, int prec,
@@ -167,12 +189,21 @@ k_raw(const sfx_cell_t *__restrict__ pts, long n, const double *__restrict__ sfi
     for(int d = 0; d < D; d++) sf[d] = sfin[i*D + d];
     (void)pts;
     if(prec == 0){ double ab = 0; double f = G::template f<double>(sf, &ab, 1); o1[i] = f; o2[i] = ab; }
-    else {
+    else if(prec == 1){
         sfx_dd sq[D];
 #pragma unroll
         for(int d = 0; d < D; d++) sq[d] = sfx_dd(sf[d]);
         sfx_dd ab; sfx_dd f = G::template f<sfx_dd>(sq, &ab, 1);
         o1[i] = f.hi; o2[i] = f.lo; o3[i] = ab.hi; o4[i] = ab.lo;
+    } else {
+        if constexpr(G::QD){
+            sfx_qd sq[D];
+#pragma unroll
+            for(int d = 0; d < D; d++) sq[d] = sfx_qd(sf[d]);
+            sfx_qd ab; sfx_qd f = G::template f<sfx_qd>(sq, &ab, 1);
+            f = sfxqd::renorm(f); ab = sfxqd::renorm(ab);
+            o1[i] = f.x[0]; o2[i] = f.x[1] + (f.x[2] + f.x[3]); o3[i] = ab.x[0]; o4[i] = ab.x[1] + (ab.x[2] + ab.x[3]);
+        }
     }
 }
 
@@ -206,7 +237,10 @@ struct sfx_graph_entry {
                      long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
     hipError_t (*raw)(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2, double *o3, double *o4);
     hipError_t (*pt)(hipStream_t s, long n, const double *xhi, const double *xlo, double *vhi, double *vlo);
+    hipError_t (*qd)(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, const uint32_t *cnt,
+                     long maxcnt, const sfx_eval_params_t *P, double *v, double *e2, uint8_t *nd);
     int nl, fixed, has_pt;     // lines, the fixed line, whether the graph has a point function (not VP)
+    int has_qd;                // whether the quad-double recompute exists (two-loop graphs)
     const void *kfn[5];        // the kernels (eval nodes, eval list AB 0, eval list AB 1, dd nodes, dd cells)
 };
 
@@ -222,6 +256,8 @@ template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 0>(SFX_EVAL_ARGS);      
 template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 1>(SFX_EVAL_ARGS);                                   \
 template __global__ void sfx::k_dd<G, false>(SFX_DD_ARGS);                                                   \
 template __global__ void sfx::k_dd<G, true>(SFX_DD_ARGS);                                                    \
+template __global__ void sfx::k_qd<G, false>(SFX_DD_ARGS);                                                   \
+template __global__ void sfx::k_qd<G, true>(SFX_DD_ARGS);                                                    \
 template __global__ void sfx::k_raw<G>(const sfx_cell_t*, long, const double*, int, double*, double*, double*, double*); \
 template __global__ void sfx::k_pt<G>(long, const double*, const double*, double*, double*);
 #else
@@ -247,6 +283,16 @@ static hipError_t sfx_dd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts,
     else hipLaunchKernelGGL((sfx::k_dd<G, false>), dim3((unsigned)nb), dim3(SFX_BLOCK), 0, s, pts, st, idx, cnt, *P, v, e2, nd);   \
     return hipGetLastError();                                                                                \
 }                                                                                                            \
+static hipError_t sfx_qd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, \
+                                const uint32_t *cnt, long maxcnt, const sfx_eval_params_t *P, double *v,     \
+                                double *e2, uint8_t *nd){                                                    \
+    if(!G::QD) return hipErrorNotSupported;                                                                  \
+    if(maxcnt <= 0) return hipSuccess;                                                                       \
+    long nb = (maxcnt + SFX_BLOCK - 1)/SFX_BLOCK; if(nb > 4096) nb = 4096;                                   \
+    if(cells) hipLaunchKernelGGL((sfx::k_qd<G, true>), dim3((unsigned)nb), dim3(SFX_BLOCK), 0, s, pts, st, idx, cnt, *P, v, e2, nd); \
+    else hipLaunchKernelGGL((sfx::k_qd<G, false>), dim3((unsigned)nb), dim3(SFX_BLOCK), 0, s, pts, st, idx, cnt, *P, v, e2, nd);   \
+    return hipGetLastError();                                                                                \
+}                                                                                                            \
 static hipError_t sfx_raw_##NAME(hipStream_t s, long n, const double *sf, int prec, double *o1, double *o2,  \
                                  double *o3, double *o4){                                                    \
     if(n <= 0) return hipSuccess;                                                                            \
@@ -262,7 +308,8 @@ static hipError_t sfx_pt_##NAME(hipStream_t s, long n, const double *xhi, const 
     return hipGetLastError();                                                                                \
 }                                                                                                            \
 extern "C" const sfx_graph_entry SFX_E_##NAME = {#NAME, VOLKOV, L, D, W,                                      \
-    sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME, sfx_pt_##NAME, G::NL, G::FIXED, G::HAS_PT ? 1 : 0,       \
+    sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME, sfx_pt_##NAME, sfx_qd_##NAME, G::NL, G::FIXED,           \
+    G::HAS_PT ? 1 : 0, G::QD ? 1 : 0,                                                                        \
     {(const void*)&sfx::k_eval<G, sfx::MODE_NODES, 2>, (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 0>,     \
      (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 1>,                                                        \
      (const void*)&sfx::k_dd<G, false>, (const void*)&sfx::k_dd<G, true>}};

@@ -359,9 +359,10 @@ static int check_graph(int graph){
 extern "C" int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out){
     int rc;
     if((rc = require_init()) || (rc = check_graph(graph))) return rc;
-    if(n < 0 || (n && (!sf || !out)) || (prec != 0 && prec != 1)){ set_error("sfx_eval_raw: bad arguments"); return SFX_EINVAL; }
+    if(n < 0 || (n && (!sf || !out)) || prec < 0 || prec > 2){ set_error("sfx_eval_raw: bad arguments"); return SFX_EINVAL; }
     if(n == 0) return SFX_OK;
     const sfx_graph_entry *g = SFX_GRAPHS[graph];
+    if(prec == 2 && !g->has_qd){ set_error("sfx_eval_raw: %s has no quad-double kernel", g->name); return SFX_EUNSUP; }
     hipStream_t s = lib().stream;
     int D = g->D, no = prec ? 4 : 2;
     DevBuf dsf, dout;
@@ -510,6 +511,47 @@ extern "C" int sfx_scale_test(int graph, double kappa, unsigned seed, int E, sfx
             out->bad++;
         }
     }
+    return SFX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// quad-double operations, element-wise (the arithmetic of the all-QD mode, exposed for its tests)
+static __host__ __device__ sfx_qd qd_apply(int op, const sfx_qd &a, const sfx_qd &b){
+    switch(op){ case 0: return a + b; case 1: return a - b; case 2: return a*b; default: return a/b; }
+}
+__global__ void k_qd_ops(int op, long n, const double *a, const double *b, double *out){
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x; if(i >= n) return;
+    sfx_qd x(a[4*i], a[4*i + 1], a[4*i + 2], a[4*i + 3]), y(b[4*i], b[4*i + 1], b[4*i + 2], b[4*i + 3]);
+    sfx_qd r = qd_apply(op, x, y);
+    for(int k = 0; k < 4; k++) out[4*i + k] = r.x[k];
+}
+extern "C" int sfx_qd_ops(int op, long n, const double *a, const double *b, double *out, int device){
+    if(op < 0 || op > 3 || n < 0 || (n && (!a || !b || !out))){ set_error("sfx_qd_ops: bad arguments"); return SFX_EINVAL; }
+    if(!device){
+        for(long i = 0; i < n; i++){
+            sfx_qd x(a[4*i], a[4*i + 1], a[4*i + 2], a[4*i + 3]), y(b[4*i], b[4*i + 1], b[4*i + 2], b[4*i + 3]);
+            sfx_qd r = qd_apply(op, x, y);
+            for(int k = 0; k < 4; k++) out[4*i + k] = r.x[k];
+        }
+        return SFX_OK;
+    }
+    int rc;
+    if((rc = require_init())) return rc;
+    if(n == 0) return SFX_OK;
+    DevBuf da, db, dout;
+    size_t bytes = (size_t)n*4*sizeof(double);
+    if((rc = da.reserve(bytes)) || (rc = db.reserve(bytes)) || (rc = dout.reserve(bytes))){ da.release(); db.release(); dout.release(); return rc; }
+    hipStream_t s = lib().stream;
+    hipError_t e = hipMemcpyAsync(da.p, a, bytes, hipMemcpyHostToDevice, s);
+    if(e == hipSuccess) e = hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, s);
+    if(e == hipSuccess){
+        hipLaunchKernelGGL(k_qd_ops, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, op, n, da.as<const double>(), db.as<const double>(), dout.as<double>());
+        e = hipGetLastError();
+    }
+    if(e == hipSuccess) e = hipMemcpyAsync(out, dout.p, bytes, hipMemcpyDeviceToHost, s);
+    if(e == hipSuccess) e = hipStreamSynchronize(s);
+    da.release(); db.release(); dout.release();
+    if(e != hipSuccess) return hip_fail(e, "sfx_qd_ops", __FILE__, __LINE__);
     return SFX_OK;
 }
 

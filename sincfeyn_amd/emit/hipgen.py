@@ -185,6 +185,8 @@ def emit(src_path, out_dir):
              '    { return f_%s<REAL>(sf, ab, sqab); }' % name,
              '    static constexpr int NL = %d, FIXED = %d;' % (desc['n'], desc['fixed']),
              '    static constexpr bool HAS_PT = %s;' % ('true' if desc['has_pt'] else 'false'),
+             '    static constexpr bool QD = %s;     // quad-double recompute (two-loop graphs)' % (
+                 'true' if desc['L'] == 2 else 'false'),
              '    template<class REAL> static __device__ __forceinline__ REAL pt(const REAL *x, REAL *ab)',
              ('    { return pt_%s<REAL>(x, (REAL)0, ab, 1); }' % name) if desc['has_pt'] else
              '    { (void)x; *ab = (REAL)0; return (REAL)0; }',

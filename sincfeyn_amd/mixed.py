@@ -12,10 +12,11 @@ mode (G2_ABSERR=0 SINCQED_ABREL=0 SINCQED_ABSKIP=0: every point with a nonzero |
 double-double, eval.c:36-38), and each integral's value is compared with its all-DD value.
 
     python -m sincfeyn_amd.mixed [--hs 1.0,0.8,0.7] [--graphs all2,all3] [--workers 16]
-                                 [--dd all|none|N:SEED] [--out FILE.jsonl]
+                                 [--dd all|none|N:SEED] [--qd] [--out FILE.jsonl]
 
 --dd N:SEED runs the all-DD comparison on N jobs drawn with splitmix64 from SEED (the full set takes
-several minutes of device time; the three-loop light-by-light integral dominates it).  One JSON line per
+several minutes of device time; the three-loop light-by-light integral dominates it).  --qd adds the
+all-QD run (every point in quad-double) of the two-loop jobs, the error reference of both.  One JSON line per
 job, then a summary line.
 """
 import argparse
@@ -27,7 +28,8 @@ import time
 from . import sfx
 from .serve import SplitMix64, default_N, percentile
 
-ALL_DD = dict(abserr0=0.0, abrel=0.0, abskip=0)
+ALL_DD = sfx.ALL_DD
+ALL_QD = sfx.ALL_QD
 
 
 def job_list(graphs, hs, eps):
@@ -65,6 +67,7 @@ def main(argv=None):
     ap.add_argument('--eps', type=float, default=1e-12)
     ap.add_argument('--workers', type=int, default=16)
     ap.add_argument('--dd', default='all', help="all, none, or N:SEED (a seeded subset of N jobs)")
+    ap.add_argument('--qd', action='store_true', help='also the all-QD run of the two-loop jobs')
     ap.add_argument('--out', default=None)
     a = ap.parse_args(argv)
     info = sfx.init(0)
@@ -103,6 +106,13 @@ def main(argv=None):
             dd[i] = sfx.wait(j)[0]
             print('#   all-DD %s h=%g done' % (G[jobs[i][0]]['name'], jobs[i][1]), file=sys.stderr, flush=True)
         t_dd = time.perf_counter() - t0
+    qd = {}
+    if a.qd:
+        pick = [i for i, j in enumerate(jobs) if G[j[0]]['L'] == 2]
+        print('# all-QD pass: %d two-loop jobs' % len(pick), file=sys.stderr, flush=True)
+        ids = [(i, sfx.submit(*jobs[i], **ALL_QD)) for i in pick]
+        for i, j in ids:
+            qd[i] = sfx.wait(j)[0]
     sfx.engine_stop()
     worst = 0.0
     for i, ((g, h, N, eps, top), r) in enumerate(zip(jobs, bt)):
@@ -115,6 +125,11 @@ def main(argv=None):
             d.update(value_all_dd=vd, nq_all_dd=dd[i]['nq'], rel_diff_all_dd=rel,
                      counts_equal_all_dd=(r['nev'], r['kept_top']) == (dd[i]['nev'], dd[i]['kept_top']))
             worst = max(worst, rel)
+        if i in qd:
+            vq, _ = sfx.graph_value(G[g], qd[i])
+            d.update(value_all_qd=vq, rel_diff_all_qd=abs(v - vq)/abs(vq) if vq != 0 else abs(v - vq))
+            if i in dd:
+                d['rel_diff_all_dd_vs_all_qd'] = abs(d['value_all_dd'] - vq)/abs(vq) if vq != 0 else abs(d['value_all_dd'] - vq)
         emit(d)
     ev = sum(r['nev'] for r in bt)
     summ = dict(summary=True, jobs=len(jobs), hs=hs, eps=a.eps, points=ev, device=info,
@@ -126,6 +141,8 @@ def main(argv=None):
         summ['all_dd'] = dict(jobs=len(dd), selection=a.dd, wall_s=t_dd,
                               points=sum(x['nev'] for x in dd.values()), dd_points=sum(x['nq'] for x in dd.values()),
                               max_rel_diff=worst)
+    if qd:
+        summ['all_qd'] = dict(jobs=len(qd), points=sum(x['nev'] for x in qd.values()))
     emit(summ)
     return 0
 

@@ -30,7 +30,7 @@ class Opts(C.Structure):
     _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
                 ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
-                ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int)]
+                ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int)]
 
 
 class Result(C.Structure):
@@ -187,6 +187,24 @@ def eval_cells(graph, h, K, cells, st, skip=None):
     _check(lib().sfx_eval_cells(graph, h, K, nc, cells.ctypes.data, None if sk is None else sk.ctypes.data, st,
                                 fw.ctypes.data, abw.ctypes.data))
     return fw, abw
+
+
+QD_OPS = {'+': 0, '-': 1, '*': 2, '/': 3}
+
+
+def qd_ops(op, a, b, device=False):
+    """element-wise quad-double a op b on (n, 4) arrays of doubles (the all-QD mode's arithmetic)"""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 4)
+    b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros_like(a)
+    L = lib()
+    L.sfx_qd_ops.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    _check(L.sfx_qd_ops(QD_OPS[op], a.shape[0], a.ctypes.data, b.ctypes.data, out.ctypes.data, 1 if device else 0))
+    return out
+
+
+ALL_QD = dict(abserr0=0.0, abrel=0.0, abskip=0, qd_fallback=1)
+ALL_DD = dict(abserr0=0.0, abrel=0.0, abskip=0)
 
 
 def scale_test(graph, kappa=24.0, seed=1, E=1):

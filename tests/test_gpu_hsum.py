@@ -102,3 +102,22 @@ def test_config1_vs_default_reference_build(sfx, oracle_ref, name):
     """against the reference's default CPU build (FMA contraction on): the same values to 1e-12 and, on
     config 1, the same counts (measured: no near-threshold decision differs there)"""
     _cmp(sfx, oracle_ref, name, 1.0, 97, 1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ALL2)
+def test_all_qd_against_all_dd(sfx, name):
+    """the all-QD run of a two-loop sum (every point recomputed in quad-double) against the all-DD run:
+    the same points, and the sums differ by no more than the all-DD run's own round column (the
+    double-double error model, eval.c:26) -- the FP64-bulk sum's distance from the all-QD one is then
+    its arithmetic error"""
+    h, N, eps = 1.0, 97, 1e-12
+    dd = sfx.hsum(name, h, N, eps, **sfx.ALL_DD)
+    qd = sfx.hsum(name, h, N, eps, **sfx.ALL_QD)
+    bulk = sfx.hsum(name, h, N, eps)
+    assert (qd['nev'], qd['kept_top'], qd['nq']) == (dd['nev'], dd['kept_top'], dd['nq'])
+    assert abs(qd['S'][0] - dd['S'][0]) <= 3*dd['unres'] + 1e-30
+    assert qd['unres'] <= 1e-50*abs(qd['S'][0]) + 1e-300
+    print(name, 'bulk - QD %.2e  DD - QD %.2e  (DD round column %.2e)' % (
+        bulk['S'][0] - qd['S'][0], dd['S'][0] - qd['S'][0], dd['unres']), flush=True)
+    assert abs(bulk['S'][0] - qd['S'][0]) <= 1e-12*abs(qd['S'][0])

@@ -109,3 +109,46 @@ def test_scale_test_matches_oracle(sfx, oracle, name, E):
 @pytest.mark.gpu
 def test_scale_test_vp_graph(sfx):
     assert sfx.scale_test('v3_00') is None
+
+
+@pytest.mark.gpu
+def test_qd_ops_device_equals_host(sfx):
+    """the device's quad-double operations are the host's, bit for bit (same code, no contraction)"""
+    rng = np.random.default_rng(5)
+    a = np.zeros((4000, 4)); b = np.zeros((4000, 4))
+    for x in (a, b):
+        x[:, 0] = rng.standard_normal(4000)*np.exp(rng.uniform(-20, 20, 4000))
+        for k in range(1, 4):
+            x[:, k] = x[:, k - 1]*2.0**-53*rng.uniform(-1, 1, 4000)
+    b[:500] = -a[:500]; b[:500, 2] *= 3.0
+    for op in '+-*/':
+        assert (sfx.qd_ops(op, a, b, device=True) == sfx.qd_ops(op, a, b)).all(), op
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name', ['g2_00', 'g2_01', 'g2_02', 'g2_03', 'v2_00'])
+def test_qd_points_against_f128(sfx, oracle, name):
+    """the device's quad-double integrand against the reference's kernels built with QREAL = __float128:
+    |f_qd - f_f128| <= 64 eps128 Sum|terms| (eps128 = 1.93e-34, common.h:15; measured up to 20: the
+    intermediate values of the generated expressions exceed Sum|terms|), four times tighter than the
+    double-double agreement, which is checked too (<= 4 QEPS Sum|terms|, measured up to 1.6)"""
+    gi = sfx.find(name)
+    D = sfx.graphs()[gi]['D']
+    sf = np.exp(np.random.default_rng(17).uniform(-4, 4, size=(2000, D)))
+    q = sfx.eval_raw(gi, sf, prec=2)
+    f = oracle.eval_fq128(name, sf)
+    d = sfx.eval_raw(gi, sf, prec=1)
+    ab = f[:, 2]
+    r128 = np.abs((q[:, 0] - f[:, 0]) + (q[:, 1] - f[:, 1]))/ab
+    rdd = np.abs((q[:, 0] - d[:, 0]) + (q[:, 1] - d[:, 1]))/ab
+    print(name, 'max |qd - f128|/ab %.3g  max |qd - dd|/ab %.3g' % (r128.max(), rdd.max()), flush=True)
+    assert (r128 <= 64*1.93e-34).all()
+    assert (rdd <= 4*4.93e-32).all()
+    assert (np.abs(q[:, 2] - ab) <= 1e-30*ab).all()
+
+
+@pytest.mark.gpu
+def test_qd_graph_without_kernel(sfx):
+    """three-loop graphs have no quad-double kernel: the mode is refused"""
+    with pytest.raises(sfx.SfxError):
+        sfx.hsum('g3_07', 1.0, 97, 1e-10, **sfx.ALL_QD)
