@@ -46,8 +46,28 @@ struct JobBufs {
 };
 // a context: one HIP stream and its device buffers; the library's own context serves sfx_hsum and the
 // per-point calls, each engine worker has its own (so jobs on different workers overlap on the device)
+// pinned host staging for the coarsest level's per-frontier transfers (asynchronous copies need it)
+struct PinBuf {
+    void *p = nullptr; size_t cap = 0;
+    int reserve(size_t bytes){
+        if(bytes <= cap) return SFX_OK;
+        size_t want = cap ? cap : 4096; while(want < bytes) want *= 2;
+        if(p) (void)hipHostFree(p);
+        p = nullptr; cap = 0;
+        if(hipHostMalloc(&p, want, 0) != hipSuccess){ p = nullptr; set_error("hipHostMalloc(%zu bytes) failed", want); return SFX_ENOMEM; }
+        cap = want;
+        return SFX_OK;
+    }
+    void release(){ if(p) (void)hipHostFree(p); p = nullptr; cap = 0; }
+    template<class T> T *as() const { return (T*)p; }
+};
+struct HostBufs {
+    PinBuf v, fl, e2, idx, misc;
+    void release(){ v.release(); fl.release(); e2.release(); idx.release(); misc.release(); }
+    void trim(size_t keep){ PinBuf *all[] = {&v, &fl, &e2, &idx, &misc}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
+};
 struct Ctx {
-    hipStream_t s = nullptr; JobBufs B; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s = nullptr; JobBufs B; HostBufs H; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 static Ctx &default_ctx(){ static Ctx C; C.s = lib().stream; return C; }
 
@@ -93,40 +113,39 @@ struct Job {
 // eval.c:33-60): v, flags (SFX_PT_*), plus the fallback count, the round column's e^2 and the
 // unresolved count, accumulated in list order
 __global__ void k_grow_shell(const sfx_cell_t *pts, long n, int K, int st, int D, uint8_t *fl);
+__global__ void k_zero_u32(uint32_t *p, long n){ for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) p[i] = 0; }
+
 
 // nodes already on the device (dpts); the values stay in B.v / B.fl and are copied to v, fl (raw flag
-// bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that step
+// bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that step.  One
+// host round trip: the values, flags and round terms come back together, and the fallback points'
+// round terms and unresolved marks are summed in node order from the flags.
 static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, double *v, uint8_t *fl, long *nq, double *u, long *nd){
     if(n <= 0) return SFX_OK;
-    JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; int rc;
-    long nb = (n + TILE - 1)/TILE;
-    if((rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) ||
-       (rc = B.ddblk.reserve((2*nb + 2)*sizeof(uint32_t))) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
-       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double)))) return rc;
+    JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; Ctx &X = *J.ctx; int rc;
+    if((rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
+       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double))) ||
+       (rc = H.v.reserve(n*sizeof(double))) || (rc = H.fl.reserve(n)) || (rc = H.e2.reserve(n*sizeof(double)))) return rc;
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(), nullptr, nullptr));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, B.ddcnt.as<uint32_t>(), 1L);
+    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(),
+                      B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
-    SFX_HIP(compact_flags(s, B.fl.as<uint8_t>(), n, SFX_FL_DD, B.ddblk.as<uint32_t>(), B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     SFX_HIP((J.o.qd_fallback ? J.g->qd : J.g->dd)(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
                     B.v.as<double>(), B.e2.as<double>(), B.fl.as<uint8_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
     if(st > 0) hipLaunchKernelGGL(k_grow_shell, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, dpts, n, J.K, st, J.D, B.fl.as<uint8_t>());
-    uint32_t m = 0;
-    SFX_HIP(hipMemcpyAsync(v, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipMemcpyAsync(fl, B.fl.p, n, hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipMemcpyAsync(&m, B.ddcnt.p, sizeof m, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(H.v.p, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(H.fl.p, B.fl.p, n, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(H.e2.p, B.e2.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
-    std::vector<double> e2; std::vector<uint32_t> idx(m);
-    if(m){
-        e2.resize(n);
-        SFX_HIP(hipMemcpyAsync(e2.data(), B.e2.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
-        SFX_HIP(hipMemcpyAsync(idx.data(), B.ddidx.p, m*sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        SFX_HIP(hipStreamSynchronize(s));
-    }
+    memcpy(v, H.v.p, n*sizeof(double)); memcpy(fl, H.fl.p, n);
+    const double *e2 = H.e2.as<double>();
+    long m = 0;
+    for(long i = 0; i < n; i++) if(fl[i] & SFX_FL_DD){ m++; *u += e2[i]; if(fl[i] & SFX_FL_UNRES) (*nd)++; }
     if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m); }
-    for(uint32_t j = 0; j < m; j++){ long i = idx[j]; *u += e2[i]; if(fl[i] & SFX_FL_UNRES) (*nd)++; }
     *nq += m;
     return SFX_OK;
 }
@@ -219,7 +238,6 @@ __global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, lon
         if(b == 1u) lsk[os++] = c; else if(b == 2u) lfu[of++] = c;
     }
 }
-__global__ void k_zero_u32(uint32_t *p, long n){ for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) p[i] = 0; }
 
 #include "sfx_ckpt.inc"
 
@@ -327,7 +345,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
 using namespace sfx;
 
 namespace sfx { void engine_shutdown(); }
-extern "C" void sfx_release_job_buffers(void){ engine_shutdown(); default_ctx().B.release(); }
+extern "C" void sfx_release_job_buffers(void){ engine_shutdown(); default_ctx().B.release(); default_ctx().H.release(); }
 
 extern "C" int sfx_prof_enable(int on){
     Prof &P = prof();
@@ -583,7 +601,7 @@ static void worker_main(int w){
         sfx_result_t r;
         int rc = hsum_core(X, j->graph, j->h, j->N, j->eps, j->top, &j->o, &r);
         (void)hipStreamSynchronize(X.s);
-        X.B.trim((size_t)256 << 20);
+        X.B.trim((size_t)256 << 20); X.H.trim((size_t)64 << 20);
         {
             std::lock_guard<std::mutex> lk(E.mu);
             j->res = r; j->rc = rc; j->t_end = now();
@@ -618,7 +636,7 @@ void engine_shutdown(){
     E.cv.notify_all();
     for(std::thread &t : E.workers) t.join();
     E.workers.clear();
-    for(Ctx *c : E.ctx){ (void)hipStreamSynchronize(c->s); c->B.release(); for(hipEvent_t &v : c->ev) if(v) (void)hipEventDestroy(v);
+    for(Ctx *c : E.ctx){ (void)hipStreamSynchronize(c->s); c->B.release(); c->H.release(); for(hipEvent_t &v : c->ev) if(v) (void)hipEventDestroy(v);
                          (void)hipStreamDestroy(c->s); delete c; }
     E.ctx.clear();
     std::lock_guard<std::mutex> lk(E.mu);

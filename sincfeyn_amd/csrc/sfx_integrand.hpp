@@ -69,8 +69,9 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
 
 // MODE_NODES: point i is record i.  MODE_LIST: the children of the cells of a list: point i is child
 // b = i mod 2^D of cell c = list[i >> D] of pts, written to slot (c << D) | b -- so a block's cells can be
-// run as two launches (cells whose children skip the |.| sums, and the others) into one value array;
-// the slots flagged for the double-double recompute are appended to ddidx[0 .. *ddcnt).
+// run as two launches (cells whose children skip the |.| sums, and the others) into one value array.
+// With ddidx, the slots flagged for the high-precision recompute are appended to ddidx[0 .. *ddcnt), in
+// no particular order.
 // AB: 0 every point skips the |.| sums (the integrand is compiled with SQ_AB = 0: those sums are dead
 // code and the register pressure drops), 1 every point computes them, 2 chosen per wave.
 constexpr int MODE_NODES = 0, MODE_LIST = 1;
@@ -115,7 +116,7 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
         }
     }
     if(in){ vout[o] = v; flout[o] = (uint8_t)fl; }
-    if(MODE == MODE_LIST && ddidx){
+    if(ddidx){
         // the points that need the double-double recompute, appended to ddidx (one atomic per wave; the
         // list's order is immaterial: the recompute writes each point at its own slot)
         bool q = (fl & SFX_FL_DD) != 0;
