@@ -85,6 +85,10 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
     long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
     if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
     if(blockIdx.x*(long)blockDim.x >= n) return;               // the whole block is past the list
+    // the job's table e^{kh} (2K+1 <= 255 values) staged in LDS: the s lookups of every point read it
+    __shared__ double sEX[256];
+    for(int j = threadIdx.x; j < 2*P.K + 1; j += blockDim.x) sEX[j] = P.EX[j];
+    __syncthreads();
     bool in = i < n;
     int k[D]; unsigned cfl = 0; bool valid = false; long o = i;
     if(in){
@@ -97,7 +101,7 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
     bool skip = valid && P.abskip && (cfl & 1u);
     double v = 0; unsigned fl = 0;
     if(valid){
-        double sf[D]; double w = lattice_node<D>(P.EX, P.K, P.H, k, sf);
+        double sf[D]; double w = lattice_node<D>(sEX, P.K, P.H, k, sf);
         double ab = 0, f;
         if(AB == 0) f = G::template f<double>(sf, &ab, 0);
         else if(AB == 1) f = G::template f<double>(sf, &ab, 1);
