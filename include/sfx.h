@@ -5,8 +5,10 @@
  * bindings (ctypes) and any other caller bind to these symbols.  Every function returns 0 on success and
  * a negative SFX_E* code on failure (never exit(), unlike the reference's CUDA back end, gpu.cu:16);
  * sfx_last_error() describes the last failure of the calling thread.  Device memory is owned by the
- * library; result structs and host arrays are owned by the caller.  Calls are made from one host
- * thread at a time.
+ * library; result structs and host arrays are owned by the caller.  The calls that run on the library's
+ * own stream (sfx_hsum, the per-point primitives, sfx_scale_test) are made from one host thread at a
+ * time; the job engine's calls (sfx_job_*) may come from any thread, and its workers run jobs
+ * concurrently, each on its own stream with its own buffers.
  *
  * Correspondence with the reference (reasther/sincFeyn v0.9.0):
  *   sfx_init / sfx_info          gpu_setup's device choice and gpu_info       (driver/gpu.cu:30-34, gpu.h:21)
@@ -18,6 +20,11 @@
  *   sfx_hsum                     hsum() with lattice_setup()                  (driver/sum.c:556-814,
  *                                lattice.c:13-27), symmetry fold off as in the reference GPU build (sum.c:571)
  *   sfx_result_t                 res_t plus the counters the reference logs   (driver/sq.h:73, main.c:53-65)
+ *   sfx_opts_t.ckpt_*            checkpoints in the SQCK4 file format        (driver/checkpoint.c:1-122)
+ *   sfx_scale_test               `sincqed scale`                              (driver/main.c:77-116)
+ *   sfx_job_* / sfx_engine_*     no counterpart: continuous batching of many sums on one device
+ *   sfx_qd_ops, qd_fallback      no counterpart in the driver: the quad-double error reference of the
+ *                                two-loop sums (the reference's pslq/quad programs compute such sums)
  */
 #ifndef SFX_H
 #define SFX_H
