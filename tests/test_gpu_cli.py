@@ -1,0 +1,57 @@
+"""The command-line host end to end on the GPU: `sincqed run 1.0 97 1e-12 all2` writes the reference's
+results-file format (driver/main.c:66-73, 226, 246), each line matching the reference's own output for
+config 1 (tests/golden/config1_sym0.txt, the reference run with SINCQED_SYM=0)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, 'sincfeyn_amd', '_lib', 'sincqed')
+GOLD = os.path.join(ROOT, 'tests', 'golden', 'config1_sym0.txt')
+
+
+def _lines(fn):
+    rec, tail = {}, []
+    for line in open(fn):
+        if line.startswith('# SUM'):
+            tail.append(line.split())
+        elif line.strip() and not line.startswith('#'):
+            rec[line.split()[0]] = line
+    return rec, tail
+
+
+def _split(line):
+    """(name, volkov, the numeric columns) of a results line: the Volkov string has blanks"""
+    name, rest = line[:8].strip(), line[9:]
+    volkov, nums = rest[:28].strip(), rest[28:].split()
+    return name, volkov, nums
+
+
+@pytest.mark.gpu
+def test_run_config1_results_file(tmp_path):
+    res, log = tmp_path / 'r.txt', tmp_path / 'log.txt'
+    env = dict(os.environ, SINCQED_RESULTS=str(res), SINCQED_LOG=str(log), SINCQED_CKPT_DIR=str(tmp_path))
+    p = subprocess.run([CLI, 'run', '1.0', '97', '1e-12', 'all2'], env=env, cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    print(p.stdout[-3000:])
+    mine, msum = _lines(res)
+    gold, gsum = _lines(GOLD)
+    assert sorted(mine) == sorted(gold) == ['g2_00', 'g2_01', 'g2_02', 'g2_03', 'v2_00']
+    assert open(res).readline() == open(GOLD).readline()                       # the header line
+    for g in gold:
+        n1, v1, a = _split(mine[g])
+        n2, v2, b = _split(gold[g])
+        assert (n1, v1) == (n2, v2)
+        assert a[:7] == b[:7], g                                                # L D w h N eps top
+        assert abs(float(a[7]) - float(b[7])) <= 1e-12*abs(float(b[7])), g     # value
+        for j in range(8, 13):                                                  # err disc prune trunc round
+            assert abs(float(a[j]) - float(b[j])) <= 2e-3*abs(float(b[j])) + 1e-16, (g, j, a[j], b[j])
+        assert abs(float(a[13]) - float(b[13])) <= 1e-8*abs(float(b[13])), g   # S_2h (8 digits printed)
+        assert a[14] == b[14], g                                                # evaluations
+        assert a[16:] == b[16:], g                                              # ref and diff columns
+    assert len(msum) == 1 and msum[0][:4] == gsum[0][:4]
+    assert abs(float(msum[0][4]) - float(gsum[0][4])) <= 1e-12*abs(float(gsum[0][4]))
+    out = p.stdout
+    assert '# sum of 5 graphs' in out and 'evals 1672228' in out
