@@ -211,7 +211,7 @@ __global__ void k_zero_acc(LevelAcc *a){ if(threadIdx.x == 0 && blockIdx.x == 0)
 // lists' order is immaterial (every child is written at its own slot).
 constexpr int SPLIT_ITEMS = 8;
 __device__ __forceinline__ unsigned wave_incl_scan(unsigned x){
-    for(int d = 1; d < 64; d <<= 1){ unsigned y = __shfl_up(x, d, 64); if(__lane_id() >= d) x += y; }
+    for(int d = 1; d < 64; d <<= 1){ unsigned y = __shfl_up(x, d, 64); if((int)__lane_id() >= d) x += y; }
     return x;
 }
 __global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, long nc, int abskip, uint32_t *lsk,

@@ -203,7 +203,8 @@ int main(int argc, char **argv){
     for(int b = a; b < argc; b++){                         /* main.c:156-168 */
         int hit = 0;
         for(int i = 0; i < ng; i++){
-            if(!sfx_graph_selected(i, argv[b])) continue; hit = 1;
+            if(!sfx_graph_selected(i, argv[b])) continue;
+            hit = 1;
             int dup = 0; for(int j = 0; j < ns; j++) dup |= sel[j] == i;
             if(!dup) sel[ns++] = i;
         }
