@@ -68,7 +68,9 @@ typedef struct {
      * replaced atomically and removed when the graph finishes; read back at the start of a sum whose h,
      * N, eps and top match, so CPU (reference) and GPU runs resume each other */
     double ckpt_every;    /* SINCQED_CKPT: seconds between checkpoints, 0 = off (the library's default) */
-    char ckpt_dir[256];   /* SINCQED_CKPT_DIR, "" = current directory */
+    char ckpt_dir[256];   /* SINCQED_CKPT_DIR, "" = current directory.  The file is named after the graph
+                             only (as the reference names it), so two sums of one graph that run at the
+                             same time -- engine jobs at different h, say -- need different directories */
     int ckpt_stop_after;  /* stop the sum with SFX_ESTOPPED right after its n-th checkpoint (0 = never):
                              a job evicted this way resumes from its file */
     int qd_fallback;      /* 1: the fallback recompute in quad-double instead of double-double (graphs with
