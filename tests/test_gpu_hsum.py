@@ -121,3 +121,35 @@ def test_all_qd_against_all_dd(sfx, name):
     print(name, 'bulk - QD %.2e  DD - QD %.2e  (DD round column %.2e)' % (
         bulk['S'][0] - qd['S'][0], dd['S'][0] - qd['S'][0], dd['unres']), flush=True)
     assert abs(bulk['S'][0] - qd['S'][0]) <= 1e-12*abs(qd['S'][0])
+
+
+EDGE = [
+    ('g2_00', 3.0, 9, 1e-3, 1),        # the smallest lattices: K = 4, a handful of cells
+    ('v2_00', 2.0, 5, 1e-6, 1),        # N = 5 -> K = 4 (rounded up to a multiple of 2^(top+1))
+    ('g2_02', 1.5, 33, 1e-2, 1),       # a loose eps: most of the grow's frontier is below threshold
+    ('g2_01', 1.0, 97, 1e-15, 1),      # a tight eps: deep support, many fallback points
+    ('g2_03', 1.0, 97, 1e-12, 3),      # top = 3: three refinement levels
+    ('v3_p4Vc', 1.0, 97, 1e-12, 1),    # identically zero: box_zero reports 0 without a grow
+    ('g3_07', 2.0, 25, 1e-10, 2),      # a three-loop graph on a coarse lattice at top = 2
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps,top', EDGE)
+def test_edge_cases(sfx, oracle, name, h, N, eps, top):
+    """small, coarse, loose, tight, deep and identically-zero sums: the oracle's counts and values"""
+    r, o = _cmp(sfx, oracle, name, h, N, eps, top)
+    if name == 'v3_p4Vc':
+        assert r['zero'] == 1 and r['nev'] == 4097 and r['S'][0] == 0
+
+
+@pytest.mark.gpu
+def test_bad_arguments_are_refused(sfx):
+    """errors are return codes (never exit(), gpu.cu:16): a lattice wider than 8-bit points, top out of
+    range, a graph index out of range"""
+    L = sfx.lib()
+    o = sfx.opts(); r = sfx.Result()
+    assert L.sfx_hsum(sfx.find('g2_00'), 0.1, 1001, 1e-12, 1, sfx.C.byref(o), sfx.C.byref(r)) == -1
+    assert b'K <= 127' in L.sfx_last_error()
+    assert L.sfx_hsum(sfx.find('g2_00'), 1.0, 97, 1e-12, 0, sfx.C.byref(o), sfx.C.byref(r)) == -1
+    assert L.sfx_hsum(9999, 1.0, 97, 1e-12, 1, sfx.C.byref(o), sfx.C.byref(r)) == -1
