@@ -76,6 +76,10 @@ typedef struct {
     int qd_fallback;      /* 1: the fallback recompute in quad-double instead of double-double (graphs with
                              a quad-double kernel: the two-loop ones; else SFX_EUNSUP).  With abserr0 = 0,
                              abrel = 0, abskip = 0 this is the all-QD error reference of a two-loop sum. */
+    int box_scan;         /* 1: SINCQED_GROW=0, the coarsest level by full-box scans instead of the grow
+                             (sum.c:166-191, 610-617): every node at step 2^(top+1) h gives the reference
+                             scale, every node at step 2^top h is summed and those with |f w| >=
+                             1e-3 eps Ref are kept; no shadow tiers (esok = 0).  (2K/2^top + 1)^D nodes. */
 } sfx_opts_t;
 
 typedef struct {

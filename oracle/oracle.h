@@ -5,6 +5,8 @@
  * It restates, single-threaded and in plain C, reasther/sincFeyn v0.9.0:
  *   driver/sum.c:556-814    hsum() with the symmetry fold, predictor, dump and histograms off
  *   driver/sumkey.h:20-461  grow() and dilate() (64-bit grid keys)
+ *   driver/sum.c:166-191,610-617 scan(): the full-box scans of SINCQED_GROW=0 (plain sums in node order,
+ *                           the reference's OpenMP reduction on one thread)
  *   driver/sum.c:346-359    box_zero()
  *   driver/eval.c:25-95     finish(), qvalue(), leafx(), eval_batch() (fd per point; fd == fv bitwise)
  *   driver/sum.c:18-20,76-81 Neumaier sums kadd/kval/kblock/kflush
@@ -23,6 +25,7 @@ typedef struct {
     double abserr0, abrel, nearf, ddslack;
     int abskip, dropunres, keepbt, btmargin;
     long maxpts;          /* nodes per refinement block: the reference CPU build's 2^18 (sum.c:643) */
+    int grow;             /* SINCQED_GROW: 1 the support-following grow, 0 full-box scans (sum.c:610-617) */
 } orc_opts_t;
 
 typedef struct {

@@ -24,7 +24,7 @@ const char *orc_error(void){ return ERR; }
 
 void orc_opts_default(orc_opts_t *o){
     o->abserr0 = 1e-15; o->abrel = 1e-3; o->nearf = 1e-3; o->ddslack = 1;   /* eval.c:7,17-18 */
-    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18;
+    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18; o->grow = 1;
 }
 
 /* prefix "G_": the kernels as the reference builds them (libsqk_ref.so); "Q_": built without FP
@@ -300,6 +300,26 @@ static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, doub
     return 0;
 }
 
+/* sum.c:166-191: every node of the step-2^lev box, id = sum_i (k_i/st + K/st) W^i (axis 0 fastest), in id
+ * order with plain sums -- the reference's OpenMP reduction run on one thread; the kept nodes (|v| >= thr)
+ * with their flag byte 0, in id order */
+static void scan(int lev, double thr, vec_t *keep, double *S, double *A, double *e1, double *e2, double *drop, long *nev){
+    int st = 1 << lev, W = 2*(K/st) + 1; long n = 1; for(int i = 0; i < D; i++) n *= W;
+    double pw = pow(st, D), s = 0, a = 0, x1 = 0, x2 = 0, dr = 0;
+    for(long id = 0; id < n; id++){
+        int k[MAXD], mx = 0; long r = id;
+        for(int i = 0; i < D; i++){ k[i] = st*((int)(r % W) - K/st); r /= W; if(abs(k[i]) > mx) mx = abs(k[i]); }
+        int nr; double v = leafx(k, 0, &nr);
+        s += v*pw; a += fabs(v)*pw;
+        if(mx == K) x1 += fabs(v)*pw; else if(mx == K - st) x2 += fabs(v)*pw;
+        if(keep){
+            if(fabs(v) >= thr){ pt_t c; memset(&c, 0, sizeof c); for(int i = 0; i < D; i++) c.k[i] = (signed char)k[i]; vpush(keep, &c); }
+            else dr += fabs(v)*pw;
+        }
+    }
+    *S = s; *A = a; *e1 = x1; *e2 = x2; *drop = dr; *nev += n;
+}
+
 /* sumkey.h:20-90: neighbours merged (skip AND, tier max), sorted by key */
 static int cmpu64(const void *a, const void *b){ uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b; return x < y ? -1 : x > y; }
 static int dilate(vec_t *cur, int st){
@@ -365,9 +385,17 @@ int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_o
     if(box_zero(top, &R->nev)){ R->zero = 1; return 0; }
     vec_t cur = {0}; double S, A, e1, e2, dr;
     long nev0 = R->nev;
-    if(grow(top, eps*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev, &R->btuns)) return -1;
+    if(O.grow){ if(grow(top, eps*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev, &R->btuns)) return -1; R->ref = A; }
+    else {
+        /* SINCQED_GROW=0 (sum.c:610-617): the reference scale from the box at step 2^(top+1) h, then the
+         * box at step 2^top h with the keep threshold (its own A only decides "identically zero" below);
+         * no shadow tiers */
+        scan(top + 1, 0, NULL, &S, &A, &e1, &e2, &dr, &R->nev);
+        R->ref = A;
+        scan(top, eps*R->ref*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev);
+    }
     R->nev_level[top] = R->nev - nev0;
-    R->nq = NQ; R->ref = A; R->prune = dr; R->unres = U;
+    R->nq = NQ; R->prune = dr; R->unres = U;
     if(A == 0){ free(cur.p); R->unresdrop = UNRESDROP; return 0; }
     { double r = e2 > 0 ? e1/e2 : 0; R->trunc = (r > 0 && r < 1) ? e1*r/(1 - r) : e1; }
     double Slev[16]; int nl = 0; Slev[nl++] = S;

@@ -99,7 +99,7 @@ def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None):
         raise RuntimeError('no integrand kernels to build')
     reg = write_registry(names)
     hdrs = [os.path.join(CSRC, h) for h in ('sfx_integrand.hpp', 'sfx_dd.hpp', 'sfx_qd.hpp', 'sfx_abi_types.h')]
-    rt_hdrs = hdrs + [os.path.join(CSRC, 'sfx_device.hpp'), os.path.join(CSRC, 'sfx_grow.inc'), os.path.join(CSRC, 'sfx_ckpt.inc'), os.path.join(ROOT, 'include', 'sfx.h')]
+    rt_hdrs = hdrs + [os.path.join(CSRC, 'sfx_device.hpp'), os.path.join(CSRC, 'sfx_grow.inc'), os.path.join(CSRC, 'sfx_scan.inc'), os.path.join(CSRC, 'sfx_ckpt.inc'), os.path.join(ROOT, 'include', 'sfx.h')]
     units = [(os.path.join(GEN, 'k_%s.hip' % n), os.path.join(OBJ, 'k_%s.o' % n), hdrs + [os.path.join(GEN, 'k_%s.inc' % n)])
              for n in names]
     units.append((reg, os.path.join(OBJ, 'sfx_registry.o'), hdrs))

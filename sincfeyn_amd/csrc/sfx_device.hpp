@@ -71,4 +71,23 @@ struct LevelArgs {
 };
 hipError_t level_reduce(hipStream_t s, const LevelArgs &a);
 
+// the sums and the kept nodes of one block of a full-box scan (reference sum.c:166-191): over the n
+// nodes of the block, S += v pw, A += |v| pw, the outer shells' |v| pw (flag bytes SFX_FL_SHELL1/2, in
+// LevelAcc::b1/b2), and with keep, the dropped weight (|v| < thr) and the count of kept nodes (|v| >= thr);
+// the fallback points' e2 and unresolved marks as in level_reduce.  Added to *acc in a fixed order;
+// keepoff receives each tile's first output slot (from acc->nkept before the call).
+struct ScanArgs {
+    const double *v; const uint8_t *fl; const double *e2; const sfx_cell_t *nodes; long n;
+    double pw, thr; int keep;
+    sfx_cell_t *out; long out_cap;                  // the kept nodes' list (scan_keep)
+    LevelPart *part; uint32_t *keepoff;             // scratch: ceil(n/TILE) entries each
+    LevelAcc *acc;
+};
+hipError_t scan_sums(hipStream_t s, const ScanArgs &a);
+// the kept nodes of the block (flag byte 0, as the reference pushes them) to out[keepoff ...], in node
+// order within the block; slots at or past out_cap are not written
+hipError_t scan_keep(hipStream_t s, const ScanArgs &a);
+#define SFX_FL_SHELL1 32u      // max_d |k_d| == K        (the reference's e1 shell)
+#define SFX_FL_SHELL2 64u      // max_d |k_d| == K - st   (e2 shell)
+
 }  // namespace sfx

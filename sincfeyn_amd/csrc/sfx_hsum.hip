@@ -166,6 +166,7 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
 struct GrowOut {
     std::vector<sfx_cell_t> keep;
     double S = 0, A = 0, e1 = 0, e2 = 0, drop = 0, u = 0; long nev = 0, nq = 0, nd = 0, btuns = 0;
+    double ref = 0;                 // the reference scale Ref: A of the grow, of the coarser box for the box scans
 };
 
 // box_zero (reference sum.c:346-359): z = 0 and 4096 pseudo-random nodes of the step-2^lev lattice; the
@@ -397,6 +398,7 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
 }
 
 #include "sfx_grow.inc"
+#include "sfx_scan.inc"
 
 static int check_job(int graph, double h, int N, double eps, int top){
     if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("bad graph index %d", graph); return SFX_EINVAL; }
@@ -460,10 +462,11 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         if((rc = box_zero(J, top, &zero, &res->nev, &res->unresdrop))) return rc;
         if(zero){ res->zero = 1; res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
         GrowOut G;
-        if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc;
+        if(J.o.box_scan){ if((rc = scan_top(J, top, eps, G, &ncur))) return rc; }
+        else { if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc; G.ref = G.A; }
         res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
         res->nev_level[top] = G.nev;
-        res->ref = ref = G.A; res->prune = G.drop;
+        res->ref = ref = G.ref; res->prune = G.drop;
         double t1 = now(); res->t_grow = t1 - t0;
         if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
         { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
@@ -474,7 +477,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     double abserr = fmax(J.o.abserr0, J.o.abrel*eps*ref);          // sum.c:651
     res->abserr = abserr;
     set_params(J, abserr);
-    res->esok = resumed ? C.esok : J.D < 12;
+    res->esok = resumed ? C.esok : J.D < 12 && !J.o.box_scan;     // the box scans set no shadow tiers (sum.c:614)
     CkRun ckr; ckr.ck = &ck; ckr.res = res; ckr.Slev = Slev; ckr.t0 = t0;
     double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
     budget_acquire(held, 0);

@@ -122,10 +122,39 @@ hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask,
 // refinement block: partial sums per tile
 __device__ __forceinline__ int tier(double f, double t){ return (f >= 10*t) + (f >= 100*t); }
 
+// a tile's partial from every thread's: fixed reduction tree (wave shuffles, then the 4 waves in
+// order), written to part[blockIdx.x] and the kept count to keepoff[blockIdx.x]
+__device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, uint32_t *keepoff){
+    for(int d = 32; d > 0; d >>= 1){
+        dacc_addd(t.s[0], t.s[1], shfl_down(t.s[0], d), shfl_down(t.s[1], d));
+        dacc_addd(t.a[0], t.a[1], shfl_down(t.a[0], d), shfl_down(t.a[1], d));
+        dacc_addd(t.b1[0], t.b1[1], shfl_down(t.b1[0], d), shfl_down(t.b1[1], d));
+        dacc_addd(t.b2[0], t.b2[1], shfl_down(t.b2[0], d), shfl_down(t.b2[1], d));
+        dacc_addd(t.drop[0], t.drop[1], shfl_down(t.drop[0], d), shfl_down(t.drop[1], d));
+        dacc_addd(t.u[0], t.u[1], shfl_down(t.u[0], d), shfl_down(t.u[1], d));
+        t.nv += __shfl_down(t.nv, d, 64); t.nskip += __shfl_down(t.nskip, d, 64); t.nkeep += __shfl_down(t.nkeep, d, 64);
+        t.nq += __shfl_down(t.nq, d, 64); t.nd += __shfl_down(t.nd, d, 64);
+    }
+    __shared__ LevelPart sh[TILE_THREADS/64];
+    if(__lane_id() == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        LevelPart r = sh[0];
+        for(int w = 1; w < TILE_THREADS/64; w++){
+            const LevelPart &q = sh[w];
+            dacc_addd(r.s[0], r.s[1], q.s[0], q.s[1]); dacc_addd(r.a[0], r.a[1], q.a[0], q.a[1]);
+            dacc_addd(r.b1[0], r.b1[1], q.b1[0], q.b1[1]); dacc_addd(r.b2[0], r.b2[1], q.b2[0], q.b2[1]);
+            dacc_addd(r.drop[0], r.drop[1], q.drop[0], q.drop[1]); dacc_addd(r.u[0], r.u[1], q.u[0], q.u[1]);
+            r.nv += q.nv; r.nskip += q.nskip; r.nkeep += q.nkeep; r.nq += q.nq; r.nd += q.nd;
+        }
+        part[blockIdx.x] = r;
+        keepoff[blockIdx.x] = r.nkeep;
+    }
+}
+
 __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
     long base = (long)blockIdx.x*TILE;
-    double s[2] = {0, 0}, ab[2] = {0, 0}, b1[2] = {0, 0}, b2[2] = {0, 0}, dr[2] = {0, 0}, u[2] = {0, 0};
-    unsigned nv = 0, nsk = 0, nk = 0, nq = 0, nd = 0;
+    LevelPart t = {};
     for(int j = 0; j < TILE_ITEMS; j++){
         long i = base + j*TILE_THREADS + threadIdx.x;
         if(i >= a.n) break;
@@ -133,44 +162,14 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
         if(!(f & SFX_FL_VALID)) continue;
         double v = a.v[i], x = v*a.pw, av = fabs(v);
         unsigned cfl = a.cells[i >> a.D].fl;
-        nv++; nsk += (a.abskip && (cfl & 1u)) ? 1u : 0u;
-        if(f & SFX_FL_DD){ nq++; nd += (f & SFX_FL_UNRES) ? 1u : 0u; dacc_add(u[0], u[1], a.e2[i]); }
-        dacc_add(s[0], s[1], x); dacc_add(ab[0], ab[1], av);
-        if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(b1[0], b1[1], x); if(te >= 2) dacc_add(b2[0], b2[1], x); }
-        else if(av >= a.thr) nk++;
-        else dacc_add(dr[0], dr[1], av*a.pw);
+        t.nv++; t.nskip += (a.abskip && (cfl & 1u)) ? 1u : 0u;
+        if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
+        dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
+        if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
+        else if(av >= a.thr) t.nkeep++;
+        else dacc_add(t.drop[0], t.drop[1], av*a.pw);
     }
-    // fixed reduction tree: wave shuffles, then the 4 waves in order
-    for(int d = 32; d > 0; d >>= 1){
-        dacc_addd(s[0], s[1], shfl_down(s[0], d), shfl_down(s[1], d));
-        dacc_addd(ab[0], ab[1], shfl_down(ab[0], d), shfl_down(ab[1], d));
-        dacc_addd(b1[0], b1[1], shfl_down(b1[0], d), shfl_down(b1[1], d));
-        dacc_addd(b2[0], b2[1], shfl_down(b2[0], d), shfl_down(b2[1], d));
-        dacc_addd(dr[0], dr[1], shfl_down(dr[0], d), shfl_down(dr[1], d));
-        dacc_addd(u[0], u[1], shfl_down(u[0], d), shfl_down(u[1], d));
-        nv += __shfl_down(nv, d, 64); nsk += __shfl_down(nsk, d, 64); nk += __shfl_down(nk, d, 64);
-        nq += __shfl_down(nq, d, 64); nd += __shfl_down(nd, d, 64);
-    }
-    __shared__ LevelPart sh[TILE_THREADS/64];
-    if(__lane_id() == 0){
-        LevelPart &p = sh[threadIdx.x >> 6];
-        p.s[0] = s[0]; p.s[1] = s[1]; p.a[0] = ab[0]; p.a[1] = ab[1]; p.b1[0] = b1[0]; p.b1[1] = b1[1];
-        p.b2[0] = b2[0]; p.b2[1] = b2[1]; p.drop[0] = dr[0]; p.drop[1] = dr[1]; p.u[0] = u[0]; p.u[1] = u[1];
-        p.nv = nv; p.nskip = nsk; p.nkeep = nk; p.nq = nq; p.nd = nd;
-    }
-    __syncthreads();
-    if(threadIdx.x == 0){
-        LevelPart t = sh[0];
-        for(int w = 1; w < TILE_THREADS/64; w++){
-            const LevelPart &q = sh[w];
-            dacc_addd(t.s[0], t.s[1], q.s[0], q.s[1]); dacc_addd(t.a[0], t.a[1], q.a[0], q.a[1]);
-            dacc_addd(t.b1[0], t.b1[1], q.b1[0], q.b1[1]); dacc_addd(t.b2[0], t.b2[1], q.b2[0], q.b2[1]);
-            dacc_addd(t.drop[0], t.drop[1], q.drop[0], q.drop[1]); dacc_addd(t.u[0], t.u[1], q.u[0], q.u[1]);
-            t.nv += q.nv; t.nskip += q.nskip; t.nkeep += q.nkeep; t.nq += q.nq; t.nd += q.nd;
-        }
-        a.part[blockIdx.x] = t;
-        a.keepoff[blockIdx.x] = t.nkeep;
-    }
+    tile_store(t, a.part, a.keepoff);
 }
 
 // one block of CB threads: the tiles' partials in tile order, the kept counts scanned into offsets;
@@ -259,6 +258,56 @@ hipError_t level_reduce(hipStream_t s, const LevelArgs &a){
     hipLaunchKernelGGL(k_level_partials, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
     hipLaunchKernelGGL(k_level_combine, dim3(1), dim3(CB), 0, s, a, nb);
     if(!a.last) hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------
+// full-box scan block (sum.c:166-191): every node is a lattice node; the sums of |v| pw over the outer
+// shells go where a refinement level keeps its shadow-tier sums
+__global__ void __launch_bounds__(TILE_THREADS) k_scan_partials(ScanArgs a){
+    long base = (long)blockIdx.x*TILE;
+    LevelPart t = {};
+    for(int j = 0; j < TILE_ITEMS; j++){
+        long i = base + j*TILE_THREADS + threadIdx.x;
+        if(i >= a.n) break;
+        uint8_t f = a.fl[i];
+        double v = a.v[i], av = fabs(v)*a.pw;
+        t.nv++;
+        if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
+        dacc_add(t.s[0], t.s[1], v*a.pw); dacc_add(t.a[0], t.a[1], av);
+        if(f & SFX_FL_SHELL1) dacc_add(t.b1[0], t.b1[1], av); else if(f & SFX_FL_SHELL2) dacc_add(t.b2[0], t.b2[1], av);
+        if(a.keep){ if(fabs(v) >= a.thr) t.nkeep++; else dacc_add(t.drop[0], t.drop[1], av); }
+    }
+    tile_store(t, a.part, a.keepoff);
+}
+
+__global__ void __launch_bounds__(TILE_THREADS) k_scan_scatter(ScanArgs a){
+    __shared__ unsigned sh[TILE_THREADS/64];
+    long base = (long)blockIdx.x*TILE;
+    unsigned run = a.keepoff[blockIdx.x];
+    for(int j = 0; j < TILE_ITEMS; j++){
+        long i = base + j*TILE_THREADS + threadIdx.x;
+        bool p = i < a.n && fabs(a.v[i]) >= a.thr;
+        unsigned tot, r = block_rank(p, sh, &tot);
+        if(p && (long)(run + r) < a.out_cap){ sfx_cell_t c = a.nodes[i]; c.fl = 0; a.out[run + r] = c; }
+        run += tot;
+    }
+}
+
+hipError_t scan_sums(hipStream_t s, const ScanArgs &a){
+    long nb = (a.n + TILE - 1)/TILE;
+    if(nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_partials, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
+    LevelArgs c = {};
+    c.part = a.part; c.keepoff = a.keepoff; c.acc = a.acc;       // all that the combine step reads
+    hipLaunchKernelGGL(k_level_combine, dim3(1), dim3(CB), 0, s, c, nb);
+    return hipGetLastError();
+}
+
+hipError_t scan_keep(hipStream_t s, const ScanArgs &a){
+    long nb = (a.n + TILE - 1)/TILE;
+    if(nb == 0 || !a.keep) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_scatter, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

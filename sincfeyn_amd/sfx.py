@@ -30,7 +30,8 @@ class Opts(C.Structure):
     _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
                 ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
-                ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int)]
+                ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int),
+                ('box_scan', C.c_int)]
 
 
 class Result(C.Structure):

@@ -163,6 +163,7 @@ int main(int argc, char **argv){
     o.abskip = envi("SINCQED_ABSKIP", o.abskip); o.nearf = envd("SINCQED_NEARF", o.nearf);
     o.ddslack = envd("SINCQED_DDSLACK", o.ddslack); o.dropunres = envi("SINCQED_DROPUNRES", o.dropunres);
     o.keepbt = getenv("SINCQED_KEEPBT") != NULL; o.btmargin = envi("SINCQED_BTMARGIN", o.btmargin);
+    o.box_scan = envi("SINCQED_GROW", 1) == 0;            /* main.c:183: full-box scans instead of the grow */
     o.ckpt_every = envd("SINCQED_CKPT", 600);              /* checkpoint.c:9, main.c:184 */
     { const char *d = getenv("SINCQED_CKPT_DIR"); if(d) snprintf(o.ckpt_dir, sizeof o.ckpt_dir, "%s", d); }
     int ng = sfx_graph_count();

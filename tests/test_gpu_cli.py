@@ -55,3 +55,20 @@ def test_run_config1_results_file(tmp_path):
     assert abs(float(msum[0][4]) - float(gsum[0][4])) <= 1e-12*abs(float(gsum[0][4]))
     out = p.stdout
     assert '# sum of 5 graphs' in out and 'evals 1672228' in out
+
+
+@pytest.mark.gpu
+def test_run_box_scan_env(tmp_path, oracle):
+    """SINCQED_GROW=0 reaches the library (main.c:183): the evaluations column is the oracle's count for
+    the full-box scans, which differs from the grow's"""
+    res = tmp_path / 'r.txt'
+    env = dict(os.environ, SINCQED_RESULTS=str(res), SINCQED_LOG='-', SINCQED_CKPT='0', SINCQED_GROW='0')
+    p = subprocess.run([CLI, 'run', '1.5', '33', '1e-10', '1', 'g2_00', 'v2_00'], env=env, cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    mine, _ = _lines(res)
+    for g in ('g2_00', 'v2_00'):
+        o = oracle.hsum(g, 1.5, 33, 1e-10, 1, grow=0)
+        _, _, a = _split(mine[g])
+        assert int(a[14]) == o['nev'], g
+        assert int(a[14]) != oracle.hsum(g, 1.5, 33, 1e-10, 1)['nev'], g

@@ -153,3 +153,82 @@ def test_bad_arguments_are_refused(sfx):
     assert b'K <= 127' in L.sfx_last_error()
     assert L.sfx_hsum(sfx.find('g2_00'), 1.0, 97, 1e-12, 0, sfx.C.byref(o), sfx.C.byref(r)) == -1
     assert L.sfx_hsum(9999, 1.0, 97, 1e-12, 1, sfx.C.byref(o), sfx.C.byref(r)) == -1
+
+
+SCAN = [
+    ('g2_00', 1.5, 33, 1e-10, 1),
+    ('g2_03', 1.5, 33, 1e-10, 1),
+    ('v2_00', 1.5, 33, 1e-10, 1),
+    ('g2_01', 1.5, 33, 1e-10, 2),
+    ('v3_00', 1.5, 33, 1e-10, 2),
+    ('g2_02', 1.0, 97, 1e-12, 1),     # config 1's widest support: 113k kept nodes (the list grows)
+]
+
+
+def _cmp_scan(sfx, oracle, name, h, N, eps, top, **kw):
+    r = sfx.hsum(name, h, N, eps, top, box_scan=1, **kw)
+    o = oracle.hsum(name, h, N, eps, top, grow=0)
+    print(name, h, N, eps, top, 'box scan: device S_h %.17g  oracle %.17g  nev %d/%d nq %d/%d kept %d/%d  t %.3fs' % (
+        r['S'][0], o['S'][0], r['nev'], o['nev'], r['nq'], o['nq'], r['kept_top'], o['kept_top'], r['t']), flush=True)
+    assert (r['nev'], r['nq'], r['kept_top'], r['unresdrop']) == (o['nev'], o['nq'], o['kept_top'], o['unresdrop'])
+    assert r['nev_level'] == o['nev_level']
+    for j in range(3):
+        a, b = r['S'][j], o['S'][j]
+        if math.isnan(b):
+            assert math.isnan(a)
+        else:
+            assert abs(a - b) <= REL*abs(b), (j, a, b)
+    # the scans' sums: double-double trees here, the plain sum in node order in the oracle, whose
+    # relative error on a sum of n terms of one sign is at most (n - 1) 2^-53 (prune, trunc and Ah are
+    # such sums: the refinement's dropped weights and |.| sums are plain sums there too)
+    for k in ('ref', 'prune', 'trunc', 'Ah'):
+        assert abs(r[k] - o[k]) <= o['nev']*2.0**-53*abs(o[k]) + 1e-300, k
+    assert r['esok'] == 0 and r['es1'] == 0 and r['es2'] == 0
+    return r, o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps,top', SCAN)
+def test_box_scan(sfx, oracle, name, h, N, eps, top):
+    """SINCQED_GROW=0: the coarsest level by full-box scans (sum.c:166-191, 610-617) instead of the grow;
+    the oracle's restatement is pinned bit for bit against the reference run on one thread"""
+    _cmp_scan(sfx, oracle, name, h, N, eps, top)
+
+
+@pytest.mark.gpu
+def test_box_scan_small_blocks(sfx, oracle):
+    """many scan blocks (4096 nodes each): the kept list grows across blocks and the block sums combine
+    in block order -- the oracle's counts and values again"""
+    _cmp_scan(sfx, oracle, 'g2_02', 1.0, 97, 1e-12, 1, block_points=1 << 12)
+
+
+@pytest.mark.gpu
+def test_box_scan_config1_vs_reference(sfx):
+    """config 1 with SINCQED_GROW=0 against the reference's default build on several threads (its
+    scan sums then have no fixed order): the same counts, values to 1e-12"""
+    import oracle as O
+    ref, wall, _ = O.run_reference(1.0, 97, 1e-12, ['all2'], env={'SINCQED_SYM': '0', 'SINCQED_GROW': '0'}, threads=8)
+    for name in ALL2:
+        r = sfx.hsum(name, 1.0, 97, 1e-12, 1, box_scan=1)
+        R = ref[name]
+        print(name, 'device %.17g ref %.17g' % (r['S'][0], R['S0']), flush=True)
+        assert (r['nev'], r['nq'], r['kept_top']) == (R['nev'], R['nq'], R['kept']), name
+        assert abs(r['S'][0] - R['S0']) <= REL*abs(R['S0'])
+        assert abs(r['S'][1] - R['S1']) <= REL*abs(R['S1'])
+
+
+@pytest.mark.gpu
+def test_box_scan_against_grow_config2(sfx):
+    """config 2's shape (h = 0.7, N = 139 -> 145, top = 2) on a D = 6 graph: 37^6 = 2.6e9 nodes in the
+    fine box.  The scan covers the whole box, the grow only the support it reaches from z = 0; both sums
+    approximate the same lattice sum, so they agree within the pruned weight of both runs"""
+    g, h, N, eps = 'g3_01', 0.7, 139, 1e-12
+    s = sfx.hsum(g, h, N, eps, box_scan=1)
+    r = sfx.hsum(g, h, N, eps)
+    d = abs(s['S'][0] - r['S'][0])
+    print(g, 'scan %.17g (nev %d, kept %d, %.2fs)  grow %.17g (nev %d, kept %d)  |diff| %.2e  prune %.2e + %.2e' % (
+        s['S'][0], s['nev'], s['kept_top'], s['t'], r['S'][0], r['nev'], r['kept_top'], d, s['prune'], r['prune']), flush=True)
+    # every node of both boxes (steps 8h and 4h) once; the kept sets differ (the scan's threshold comes
+    # from the coarser box's Ref, the grow's from its own A), so only the values are compared
+    assert s['nev_level'][2] == 19**6 + 37**6
+    assert d <= s['prune'] + r['prune']

@@ -94,6 +94,28 @@ def test_oracle_bitexact_vs_reference_more(orc, variant, h, N, eps, graphs):
             assert r['S'][j] == R[k] or (math.isnan(r['S'][j]) and math.isnan(R[k])), (g, k)
 
 
+@needs_ref
+@pytest.mark.parametrize('variant', ['ref', 'nc'])
+@pytest.mark.parametrize('h,N,eps,top,graphs', [(1.5, 33, 1e-10, 1, ['g2_00', 'g2_03', 'v2_00']),
+                                                 (1.5, 33, 1e-10, 2, ['g2_01', 'v3_00']),
+                                                 (1.0, 97, 1e-12, 1, ['g2_02'])])
+def test_oracle_scan_bitexact_vs_reference(orc, variant, h, N, eps, top, graphs):
+    """SINCQED_GROW=0, the full-box scans instead of the grow (sum.c:166-191, 610-617): the reference on one
+    thread (its OpenMP reduction is then the plain sum in node order) against the restatement, bit for bit"""
+    ref, wall, _ = O.run_reference(h, N, eps, graphs, top=top, env={'SINCQED_SYM': '0', 'SINCQED_GROW': '0'}, threads=1,
+                                   nc=(variant == 'nc'))
+    for g in graphs:
+        r = orc[variant].hsum(g, h, N, eps, top, grow=0)
+        R = ref[g]
+        assert (r['nev'], r['nq'], r['kept_top'], r['unresdrop']) == (R['nev'], R['nq'], R['kept'], R['unresdrop']), g
+        for j, k in ((0, 'S0'), (1, 'S1'), (2, 'S2')):
+            assert r['S'][j] == R[k] or (math.isnan(r['S'][j]) and math.isnan(R[k])), (g, k)
+        for k in ('Ah', 'ref', 'prune', 'trunc', 'unres'):
+            assert r[k] == R[k], (g, k)
+        # a box scan sets no shadow tiers: the final level's tier sums are empty
+        assert r['es1'] == 0 and r['es2'] == 0
+
+
 def test_golden_points_reference_kernels(orc):
     """the reference kernels as built here reproduce the committed per-point vectors exactly (so the
     checkers' kernels are the ones the fixtures were made with)"""
