@@ -169,9 +169,10 @@ int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, cons
 
 /* ---- the hierarchical sum ----
  * hsum(g, h, N, eps, top) for one graph on the device: K = (N-1)/2 rounded up to a multiple of
- * 2^(top+1) (lattice.c:16; K <= 127), top >= 1.  The coarsest level (support-following grow and the
- * dilation) runs on the host with every point evaluated on the device; each refinement level runs on
- * the device with the cell lists resident in HBM. */
+ * 2^(top+1) (lattice.c:16; K <= 127), top >= 1.  The coarsest level (the support-following grow, or
+ * the full-box scans of opts->box_scan, and the dilation) runs on the device with its tables in HBM, only
+ * the grow's order-dependent frontier sums on the host; each refinement level runs on the device with
+ * the cell lists resident in HBM. */
 int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res);
 
 /* ---- job engine: continuous batching of hierarchical sums ----
