@@ -97,7 +97,8 @@ typedef struct {
     long nev_level[16];   /* evaluations per level (index = level, top..0), grow at index top */
     double abserr;        /* the fallback threshold used by the refinement */
     int K, top, levels;   /* lattice half-width after rounding, coarsest level, levels run */
-    double t_grow, t_dilate, t_refine;   /* seconds */
+    double t_grow, t_dilate, t_refine;   /* seconds: coarsest level (box_zero, grow or scans, tiers), dilation,
+                                            refinement levels */
     int resumed;          /* the sum continued from a checkpoint: at level resume_lev, cell resume_next of */
     int resume_lev;       /*   resume_ncells (sum.c:602) */
     long resume_next, resume_ncells;

@@ -31,6 +31,8 @@ typedef struct {
 #define SFX_FL_DD    4u      /* needs the double-double recompute */
 #define SFX_FL_SKIP  8u      /* evaluated without the |.| sums */
 #define SFX_FL_UNRES 16u     /* unresolved even in double-double (set by the fallback kernel) */
+#define SFX_FL_SHELL1 32u    /* node lists at step st > 0: max_d |k_d| == K       (the e1 shell, sum.c:178) */
+#define SFX_FL_SHELL2 64u    /* node lists at step st > 0: max_d |k_d| == K - st  (the e2 shell) */
 
 #ifdef __cplusplus
 }

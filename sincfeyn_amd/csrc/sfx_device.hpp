@@ -87,7 +87,6 @@ hipError_t scan_sums(hipStream_t s, const ScanArgs &a);
 // the kept nodes of the block (flag byte 0, as the reference pushes them) to out[keepoff ...], in node
 // order within the block; slots at or past out_cap are not written
 hipError_t scan_keep(hipStream_t s, const ScanArgs &a);
-#define SFX_FL_SHELL1 32u      // max_d |k_d| == K        (the reference's e1 shell)
-#define SFX_FL_SHELL2 64u      // max_d |k_d| == K - st   (e2 shell)
+
 
 }  // namespace sfx

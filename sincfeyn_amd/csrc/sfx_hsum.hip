@@ -29,12 +29,13 @@ static double now(){ return std::chrono::duration<double>(std::chrono::steady_cl
 
 struct JobBufs {
     DevBuf EX, pts, v, fl, ddblk, ddidx, ddcnt, e2, nd, cur, nxt, part, keepoff, acc;
+    DevBuf vfe;                             // a node list's values, round terms and flag bytes, back to back
     DevBuf lsk, lfu, lcnt;                  // a refinement block's cells split by |.|-sum class; counts
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
     DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
            dkey, dns, dtr, skey, skey2, sortt;
     template<class F> void each(F f){
-        DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc,
+        DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc, &vfe,
                          &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
                          &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &skey, &skey2, &sortt};
         for(DevBuf *b : all) f(*b);
@@ -62,9 +63,9 @@ struct PinBuf {
     template<class T> T *as() const { return (T*)p; }
 };
 struct HostBufs {
-    PinBuf v, fl, e2, idx, misc;
-    void release(){ v.release(); fl.release(); e2.release(); idx.release(); misc.release(); }
-    void trim(size_t keep){ PinBuf *all[] = {&v, &fl, &e2, &idx, &misc}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
+    PinBuf vfe, idx, misc;
+    void release(){ vfe.release(); idx.release(); misc.release(); }
+    void trim(size_t keep){ PinBuf *all[] = {&vfe, &idx, &misc}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
 };
 struct Ctx {
     hipStream_t s = nullptr; JobBufs B; HostBufs H; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -112,37 +113,38 @@ struct Job {
 // evaluate a list of nodes on the device with the per-point tests and the fallback (leafx semantics,
 // eval.c:33-60): v, flags (SFX_PT_*), plus the fallback count, the round column's e^2 and the
 // unresolved count, accumulated in list order
-__global__ void k_grow_shell(const sfx_cell_t *pts, long n, int K, int st, int D, uint8_t *fl);
 __global__ void k_zero_u32(uint32_t *p, long n){ for(long i = blockIdx.x*(long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x*blockDim.x) p[i] = 0; }
 
+// a node list's results in the job's buffer B.vfe, back to back so that one copy brings them to the host:
+// values v[0..n), round terms e2[0..n) (written for the fallback points only), flag bytes fl[0..n)
+static inline double *node_v(JobBufs &B){ return B.vfe.as<double>(); }
+static inline double *node_e2(JobBufs &B, long n){ return B.vfe.as<double>() + n; }
+static inline uint8_t *node_fl(JobBufs &B, long n){ return (uint8_t*)(B.vfe.as<double>() + 2*n); }
 
-// nodes already on the device (dpts); the values stay in B.v / B.fl and are copied to v, fl (raw flag
-// bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that step.  One
-// host round trip: the values, flags and round terms come back together, and the fallback points'
-// round terms and unresolved marks are summed in node order from the flags.
+// nodes already on the device (dpts); the results stay in B.vfe (node_v, node_fl) and are copied to v, fl
+// (raw flag bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that
+// step.  One host round trip: one copy brings back values, round terms and flags, and the fallback
+// points' round terms and unresolved marks are summed in node order from the flags.
 static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, double *v, uint8_t *fl, long *nq, double *u, long *nd){
     if(n <= 0) return SFX_OK;
     JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; Ctx &X = *J.ctx; int rc;
-    if((rc = B.v.reserve(n*sizeof(double))) || (rc = B.fl.reserve(n)) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) ||
-       (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(n*sizeof(double))) ||
-       (rc = H.v.reserve(n*sizeof(double))) || (rc = H.fl.reserve(n)) || (rc = H.e2.reserve(n*sizeof(double)))) return rc;
+    const size_t bytes = (size_t)n*(2*sizeof(double) + 1);
+    if((rc = B.vfe.reserve(bytes)) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) ||
+       (rc = H.vfe.reserve(bytes))) return rc;
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, B.ddcnt.as<uint32_t>(), 1L);
-    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, 0, &J.P, B.v.as<double>(), B.fl.as<uint8_t>(),
+    SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, st, &J.P, node_v(B), node_fl(B, n),
                       B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
     SFX_HIP((J.o.qd_fallback ? J.g->qd : J.g->dd)(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
-                    B.v.as<double>(), B.e2.as<double>(), B.fl.as<uint8_t>()));
+                    node_v(B), node_e2(B, n), node_fl(B, n)));
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
-    if(st > 0) hipLaunchKernelGGL(k_grow_shell, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, dpts, n, J.K, st, J.D, B.fl.as<uint8_t>());
-    SFX_HIP(hipMemcpyAsync(H.v.p, B.v.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipMemcpyAsync(H.fl.p, B.fl.p, n, hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipMemcpyAsync(H.e2.p, B.e2.p, n*sizeof(double), hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipMemcpyAsync(H.vfe.p, B.vfe.p, bytes, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
-    memcpy(v, H.v.p, n*sizeof(double)); memcpy(fl, H.fl.p, n);
-    const double *e2 = H.e2.as<double>();
+    const double *hv = H.vfe.as<double>(), *e2 = hv + n; const uint8_t *hf = (const uint8_t*)(hv + 2*n);
+    memcpy(v, hv, n*sizeof(double)); memcpy(fl, hf, n);
     long m = 0;
     for(long i = 0; i < n; i++) if(fl[i] & SFX_FL_DD){ m++; *u += e2[i]; if(fl[i] & SFX_FL_UNRES) (*nd)++; }
     if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m); }
@@ -167,6 +169,8 @@ struct GrowOut {
     std::vector<sfx_cell_t> keep;
     double S = 0, A = 0, e1 = 0, e2 = 0, drop = 0, u = 0; long nev = 0, nq = 0, nd = 0, btuns = 0;
     double ref = 0;                 // the reference scale Ref: A of the grow, of the coarser box for the box scans
+    // host seconds of the stages (the verbose report, and t_grow / t_dilate of the result)
+    double t_loop = 0, t_final = 0, t_tiers = 0, t_dilate = 0; long frontiers = 0, tier_steps = 0;
 };
 
 // box_zero (reference sum.c:346-359): z = 0 and 4096 pseudo-random nodes of the step-2^lev lattice; the
@@ -467,12 +471,16 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
         res->nev_level[top] = G.nev;
         res->ref = ref = G.ref; res->prune = G.drop;
-        double t1 = now(); res->t_grow = t1 - t0;
+        double t1 = now(); res->t_grow = t1 - t0 - G.t_dilate; res->t_dilate = G.t_dilate;
+        if(J.o.verbose)
+            fprintf(stderr, "  %s coarsest level: %.2f ms (%ld frontiers %.2f ms, final threshold %.2f ms, tiers %ld steps %.2f ms, "
+                    "dilation %.2f ms)\n", J.g->name, 1e3*(t1 - t0), G.frontiers, 1e3*G.t_loop, 1e3*G.t_final, G.tier_steps,
+                    1e3*G.t_tiers, 1e3*G.t_dilate);
         if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
         { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
         Slev[nl++] = G.S;
         res->kept_top = ncur;
-        t2 = now(); res->t_dilate = 0;
+        t2 = now();
     }
     double abserr = fmax(J.o.abserr0, J.o.abrel*eps*ref);          // sum.c:651
     res->abserr = abserr;

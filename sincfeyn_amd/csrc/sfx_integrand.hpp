@@ -67,7 +67,8 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
     return w;
 }
 
-// MODE_NODES: point i is record i.  MODE_LIST: the children of the cells of a list: point i is child
+// MODE_NODES: point i is record i (with st > 0 the flag byte also carries the node's outer-shell class
+// at step st: SFX_FL_SHELL1 for max_d |k_d| = K, SFX_FL_SHELL2 for K - st).  MODE_LIST: the children of the cells of a list: point i is child
 // b = i mod 2^D of cell c = list[i >> D] of pts, written to slot (c << D) | b -- so a block's cells can be
 // run as two launches (cells whose children skip the |.| sums, and the others) into one value array.
 // With ddidx, the slots flagged for the high-precision recompute are appended to ddidx[0 .. *ddcnt), in
@@ -111,6 +112,13 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
             else f = G::template f<double>(sf, &ab, 1);
         }
         v = f*w; fl = SFX_FL_VALID;
+        if(MODE == MODE_NODES && st > 0){
+            // the node's outer-shell class at step st (the grow's and the box scan's e1/e2 weights)
+            int mx = 0;
+#pragma unroll
+            for(int d = 0; d < D; d++){ int a = k[d] < 0 ? -k[d] : k[d]; mx = a > mx ? a : mx; }
+            fl |= mx == P.K ? SFX_FL_SHELL1 : mx == P.K - st ? SFX_FL_SHELL2 : 0u;
+        }
         if(skip){ fl |= SFX_FL_SKIP; if(!isfinite(v)) v = 0; }
         else {
             double bnd = ab*w*16*DBL_EPSILON/P.ddslack;            // eval.c:36, same association

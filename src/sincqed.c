@@ -70,6 +70,8 @@ static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opt
     if(R.unresdrop)
         say("# WARNING %s: %ld points unresolved even in double-double (eps*sum|terms| > |value|, not one digit left) %s; their bound is in the round column\n",
             g.name, R.unresdrop, o->dropunres ? "counted as 0 (SINCQED_DROPUNRES=0 keeps them)" : "kept as they came out");
+    if(getenv("SINCQED_TIMING") && !R.zero)                 /* sum.c:638: the stages' wall times */
+        say("# %s timing: top level %.3fs, dilate+sort %.3fs, refinement %.3fs\n", g.name, R.t_grow, R.t_dilate, R.t_refine);
     if(R.esok && !R.zero && R.levels > 1){                 /* sum.c:806-809 */
         double d1 = w*(R.S[0] - R.es1), dd2 = w*(R.es1 - R.es2), rr = d1 != 0 ? dd2/d1 : 0;
         say("# %s eps extrapolation: eps*10 would change the value by %+.3e, eps*100 by a further %+.3e (ratio %.2f)", g.name, -d1, -dd2, rr);

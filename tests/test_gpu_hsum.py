@@ -28,6 +28,12 @@ def _cmp(sfx, oracle, name, h, N, eps, top=None, rel=REL, **kw):
             assert abs(a - b) <= rel*abs(b), (j, a, b)
     assert abs(r['ref'] - o['ref']) <= 1e-13*abs(o['ref'])
     assert r['abskipped'] == o['abskipped']
+    # the shadow-tier sums (the eps-extrapolation line): the same tier sets, so against the oracle of the
+    # same arithmetic the sums differ by their summation error only, far below one tier-1 point (|v| >=
+    # 10 thr = 1e-3 eps Ref); against a differently rounded build, by what the values' tolerance allows
+    tol = 1e-15 if rel <= REL else rel
+    for k in ('es1', 'es2'):
+        assert abs(r[k] - o[k]) <= tol*o['Ah'] + 1e-300, (k, r[k], o[k])
     return r, o
 
 

@@ -66,7 +66,7 @@ def orc():
 @pytest.mark.parametrize('variant', ['ref', 'nc'])
 def test_oracle_bitexact_vs_reference_config1(orc, variant):
     """the restatement reproduces the reference implementation bit for bit (symmetry fold off)"""
-    ref, wall, _ = O.run_reference(1.0, 97, 1e-12, ['all2'], env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
+    ref, wall, ref_out = O.run_reference(1.0, 97, 1e-12, ['all2'], env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
     for g in ALL2:
         r = orc[variant].hsum(g, 1.0, 97, 1e-12, 1)
         R = ref[g]
@@ -75,6 +75,19 @@ def test_oracle_bitexact_vs_reference_config1(orc, variant):
                (R['nev'], R['nq'], R['kept'], R['skipped'], R['unresdrop']), g
         for k in ('Ah', 'ref', 'prune', 'trunc'):
             assert r[k] == R[k], (g, k)
+    _check_extrapolation(orc[variant], ref_out, 1.0, 97, 1e-12, 1, ALL2)
+
+
+def _check_extrapolation(o, stdout, h, N, eps, top, graphs):
+    """the shadow-tier sums against the reference's eps-extrapolation line (sum.c:806-809, printed to four
+    digits): d1 = w (S_h - ES1), d2 = w (ES1 - ES2)"""
+    lines = {l.split()[1]: l for l in stdout.splitlines() if 'eps extrapolation:' in l}
+    for g in graphs:
+        r = o.hsum(g, h, N, eps, top)
+        w = o.graph(g)['weight']
+        d1, d2 = w*(r['S'][0] - r['es1']), w*(r['es1'] - r['es2'])
+        want = 'eps*10 would change the value by %+.3e, eps*100 by a further %+.3e' % (-d1, -d2)
+        assert want in lines[g], (g, want, lines[g])
 
 
 @needs_ref
@@ -85,13 +98,14 @@ def test_oracle_bitexact_vs_reference_more(orc, variant, h, N, eps, graphs):
     """three-loop VP graphs (incl. the value-branching Pi4 contact and the identically zero v3_p4Vc) and
     top = 2 lattices"""
     top = O.default_top(h)
-    ref, wall, _ = O.run_reference(h, N, eps, graphs, env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
+    ref, wall, ref_out = O.run_reference(h, N, eps, graphs, env={'SINCQED_SYM': '0'}, threads=4, nc=(variant == 'nc'))
     for g in graphs:
         r = orc[variant].hsum(g, h, N, eps, top)
         R = ref[g]
         assert (r['nev'], r['nq'], r['kept_top']) == (R['nev'], R['nq'], R['kept']), g
         for j, k in ((0, 'S0'), (1, 'S1'), (2, 'S2')):
             assert r['S'][j] == R[k] or (math.isnan(r['S'][j]) and math.isnan(R[k])), (g, k)
+    _check_extrapolation(orc[variant], ref_out, h, N, eps, top, [g for g in graphs if ref[g]['S0'] != 0])
 
 
 @needs_ref
