@@ -238,3 +238,28 @@ def test_box_scan_against_grow_config2(sfx):
     # from the coarser box's Ref, the grow's from its own A), so only the values are compared
     assert s['nev_level'][2] == 19**6 + 37**6
     assert d <= s['prune'] + r['prune']
+
+
+@pytest.mark.gpu
+def test_config2_full_size_vs_reference_nc(sfx):
+    """config 2 at its full size (h = 0.7, N = 139 -> 145, top = 2, eps = 1e-12) for the integrals the
+    reference implementation, built without FP contraction and run on the host's cores, finishes in
+    seconds (three VP insertions and two three-loop vertex graphs, D = 5-6; g3_01 evaluates 4.7e8 points,
+    1.6% of them in double-double): identical evaluation, fallback
+    and kept counts, values to 1e-12, the shadow-tier sums through the reference's own eps-extrapolation
+    line (printed to four digits)"""
+    import oracle as O
+    graphs = ['v3_00', 'v3_03', 'v3_04', 'g3_07', 'g3_01']
+    ref, wall, out = O.run_reference(0.7, 139, 1e-12, graphs, env={'SINCQED_SYM': '0'}, nc=True)
+    lines = {l.split()[1]: l for l in out.splitlines() if 'eps extrapolation:' in l}
+    for g in graphs:
+        r = sfx.hsum(g, 0.7, 139, 1e-12)
+        R = ref[g]
+        print(g, 'device %.17g ref %.17g nev %d/%d nq %d/%d kept %d/%d (reference %.1fs for all)' % (
+            r['S'][0], R['S0'], r['nev'], R['nev'], r['nq'], R['nq'], r['kept_top'], R['kept'], wall), flush=True)
+        assert (r['nev'], r['nq'], r['kept_top'], r['abskipped']) == (R['nev'], R['nq'], R['kept'], R['skipped']), g
+        for j, k in ((0, 'S0'), (1, 'S1'), (2, 'S2')):
+            assert abs(r['S'][j] - R[k]) <= REL*abs(R[k]), (g, k)
+        w = sfx.graphs()[sfx.find(g)]['weight']
+        d1, d2 = w*(r['S'][0] - r['es1']), w*(r['es1'] - r['es2'])
+        assert 'eps*10 would change the value by %+.3e, eps*100 by a further %+.3e' % (-d1, -d2) in lines[g], (g, lines[g])
