@@ -85,3 +85,21 @@ def test_default_context_after_tight_engine(sfx):
     sfx.engine_stop()
     r = sfx.hsum('g2_03', 0.7, 139, 1e-12)
     assert r['nev'] > 0
+
+
+@pytest.mark.gpu
+def test_mix_with_box_scans_and_all_dd(sfx):
+    """jobs with other options in the same mix -- full-box scans (SINCQED_GROW=0) and the all-DD error
+    reference -- beside default jobs: each result identical to the same job alone"""
+    cases = [('g2_02', 1.0, 97, 1e-12, dict(box_scan=1)), ('g2_00', 1.0, 97, 1e-12, {}),
+             ('g2_01', 1.0, 97, 1e-12, dict(sfx.ALL_DD)), ('v3_00', 1.5, 33, 1e-10, dict(box_scan=1)),
+             ('g3_07', 1.0, 97, 1e-10, {}), ('v2_00', 1.0, 97, 1e-12, dict(box_scan=1, **sfx.ALL_DD))]
+    alone = [sfx.hsum(g, h, N, eps, **kw) for g, h, N, eps, kw in cases]
+    sfx.engine_stop()
+    sfx.engine_start(4, 0)
+    jobs = [sfx.submit(g, h, N, eps, **kw) for g, h, N, eps, kw in cases]
+    mixed = [sfx.wait(j)[0] for j in jobs]
+    for (g, h, N, eps, kw), a, m in zip(cases, alone, mixed):
+        assert _same(a, m), (g, kw, a['S'], m['S'])
+        assert (a['esok'], a['es1'], a['es2']) == (m['esok'], m['es1'], m['es2']), (g, kw)
+    sfx.engine_stop()
