@@ -66,7 +66,7 @@ struct LevelArgs {
     double pw, thr;
     const double *e2;
     sfx_cell_t *nxt; long nxt_cap;
-    LevelPart *part; uint32_t *keepoff;             // scratch: ceil(n/TILE) entries each
+    LevelPart *part; unsigned long long *keepoff;   // scratch: ceil(n/TILE) entries each
     LevelAcc *acc;
 };
 hipError_t level_reduce(hipStream_t s, const LevelArgs &a);
@@ -80,7 +80,7 @@ struct ScanArgs {
     const double *v; const uint8_t *fl; const double *e2; const sfx_cell_t *nodes; long n;
     double pw, thr; int keep;
     sfx_cell_t *out; long out_cap;                  // the kept nodes' list (scan_keep)
-    LevelPart *part; uint32_t *keepoff;             // scratch: ceil(n/TILE) entries each
+    LevelPart *part; unsigned long long *keepoff;   // scratch: ceil(n/TILE) entries each
     LevelAcc *acc;
 };
 hipError_t scan_sums(hipStream_t s, const ScanArgs &a);

@@ -261,7 +261,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     long nmax = Bc << D, nbt = (nmax + TILE - 1)/TILE;
     if((rc = B.v.reserve(nmax*sizeof(double))) || (rc = B.fl.reserve(nmax)) || (rc = B.ddblk.reserve((2*nbt + 2)*sizeof(uint32_t))) ||
        (rc = B.ddidx.reserve(nmax*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) || (rc = B.e2.reserve(nmax*sizeof(double))) ||
-       (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(uint32_t))) ||
+       (rc = B.nd.reserve(nmax)) || (rc = B.part.reserve(nbt*sizeof(LevelPart))) || (rc = B.keepoff.reserve(nbt*sizeof(unsigned long long))) ||
        (rc = B.acc.reserve(sizeof(LevelAcc))) || (rc = B.lsk.reserve(Bc*sizeof(uint32_t))) ||
        (rc = B.lfu.reserve(Bc*sizeof(uint32_t)))) return rc;
     // the next level's list at its bound (every child kept; the job's HBM reservation covers it), so the
@@ -301,7 +301,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         LevelArgs a;
         a.v = B.v.as<double>(); a.fl = B.fl.as<uint8_t>(); a.cells = cells; a.n = n; a.D = D; a.st = st; a.last = last;
         a.abskip = J.o.abskip; a.pw = pw; a.thr = thr; a.e2 = B.e2.as<double>(); a.nxt = B.nxt.as<sfx_cell_t>(); a.nxt_cap = (long)(B.nxt.cap/sizeof(sfx_cell_t));
-        a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<uint32_t>(); a.acc = B.acc.as<LevelAcc>();
+        a.part = B.part.as<LevelPart>(); a.keepoff = B.keepoff.as<unsigned long long>(); a.acc = B.acc.as<LevelAcc>();
         SFX_HIP(level_reduce(s, a));
         if(pon){
             SFX_HIP(hipEventRecord(X.ev[5], s));

@@ -124,7 +124,7 @@ __device__ __forceinline__ int tier(double f, double t){ return (f >= 10*t) + (f
 
 // a tile's partial from every thread's: fixed reduction tree (wave shuffles, then the 4 waves in
 // order), written to part[blockIdx.x] and the kept count to keepoff[blockIdx.x]
-__device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, uint32_t *keepoff){
+__device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, unsigned long long *keepoff){
     for(int d = 32; d > 0; d >>= 1){
         dacc_addd(t.s[0], t.s[1], shfl_down(t.s[0], d), shfl_down(t.s[1], d));
         dacc_addd(t.a[0], t.a[1], shfl_down(t.a[0], d), shfl_down(t.a[1], d));
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
     }
     {
         unsigned long long ex = shc[0][threadIdx.x] - nk + a.acc->nkept;
-        for(long b = b0; b < b1; b++){ a.keepoff[b] = (uint32_t)ex; ex += a.part[b].nkeep; }
+        for(long b = b0; b < b1; b++){ a.keepoff[b] = ex; ex += a.part[b].nkeep; }
     }
     __syncthreads();
     for(int k = 0; k < 6; k++){ sh[k][0][threadIdx.x] = v[k][0]; sh[k][1][threadIdx.x] = v[k][1]; }
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
 __global__ void __launch_bounds__(TILE_THREADS) k_level_scatter(LevelArgs a){
     __shared__ unsigned sh[TILE_THREADS/64];
     long base = (long)blockIdx.x*TILE;
-    unsigned run = a.keepoff[blockIdx.x];
+    unsigned long long run = a.keepoff[blockIdx.x];
     int D = a.D, msk = (1 << D) - 1;
     for(int j = 0; j < TILE_ITEMS; j++){
         long i = base + j*TILE_THREADS + threadIdx.x;
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_scan_partials(ScanArgs a){
 __global__ void __launch_bounds__(TILE_THREADS) k_scan_scatter(ScanArgs a){
     __shared__ unsigned sh[TILE_THREADS/64];
     long base = (long)blockIdx.x*TILE;
-    unsigned run = a.keepoff[blockIdx.x];
+    unsigned long long run = a.keepoff[blockIdx.x];
     for(int j = 0; j < TILE_ITEMS; j++){
         long i = base + j*TILE_THREADS + threadIdx.x;
         bool p = i < a.n && fabs(a.v[i]) >= a.thr;
