@@ -63,13 +63,38 @@ struct PinBuf {
     template<class T> T *as() const { return (T*)p; }
 };
 struct HostBufs {
-    PinBuf vfe, idx, misc;
-    void release(){ vfe.release(); idx.release(); misc.release(); }
-    void trim(size_t keep){ PinBuf *all[] = {&vfe, &idx, &misc}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
+    PinBuf vfe, idx, misc, rb, up;       // rb, up: staging of the small read-backs and uploads
+    void release(){ vfe.release(); idx.release(); misc.release(); rb.release(); up.release(); }
+    void trim(size_t keep){ PinBuf *all[] = {&vfe, &idx, &misc, &rb, &up}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
 };
 struct Ctx {
     hipStream_t s = nullptr; JobBufs B; HostBufs H; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
+
+// Host <-> device transfers of the context's stream go through its pinned staging, never through pageable
+// memory (the runtime stages such copies itself, in buffers shared by every thread of the process).
+// read_back: the n small device regions into their host destinations, one stream synchronisation.
+struct RB { void *dst; const void *src; size_t bytes; };
+static int read_back(Ctx &X, hipStream_t s, const RB *it, int n){
+    size_t tot = 0; int rc;
+    for(int i = 0; i < n; i++) tot += (it[i].bytes + 15) & ~(size_t)15;
+    if((rc = X.H.rb.reserve(tot))) return rc;
+    char *p = X.H.rb.as<char>();
+    for(size_t i = 0, off = 0; i < (size_t)n; off += (it[i].bytes + 15) & ~(size_t)15, i++)
+        SFX_HIP(hipMemcpyAsync(p + off, it[i].src, it[i].bytes, hipMemcpyDeviceToHost, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    for(size_t i = 0, off = 0; i < (size_t)n; off += (it[i].bytes + 15) & ~(size_t)15, i++) memcpy(it[i].dst, p + off, it[i].bytes);
+    return SFX_OK;
+}
+// upload: host bytes to the device through the staging, complete when it returns (the staging is reused)
+static int upload(Ctx &X, hipStream_t s, void *dst, const void *src, size_t bytes){
+    if(!bytes) return SFX_OK;
+    int rc; if((rc = X.H.up.reserve(bytes))) return rc;
+    memcpy(X.H.up.p, src, bytes);
+    SFX_HIP(hipMemcpyAsync(dst, X.H.up.p, bytes, hipMemcpyHostToDevice, s));
+    SFX_HIP(hipStreamSynchronize(s));
+    return SFX_OK;
+}
 static Ctx &default_ctx(){ static Ctx C; C.s = lib().stream; return C; }
 
 // ---- profiling: HIP events on the library stream around each kernel, accumulated per (graph, kind)
@@ -128,9 +153,11 @@ static inline uint8_t *node_fl(JobBufs &B, long n){ return (uint8_t*)(B.vfe.as<d
 static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, double *v, uint8_t *fl, long *nq, double *u, long *nd){
     if(n <= 0) return SFX_OK;
     JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; Ctx &X = *J.ctx; int rc;
-    const size_t bytes = (size_t)n*(2*sizeof(double) + 1);
-    if((rc = B.vfe.reserve(bytes)) || (rc = B.ddidx.reserve(n*sizeof(uint32_t))) || (rc = B.ddcnt.reserve(64)) ||
-       (rc = H.vfe.reserve(bytes))) return rc;
+    // at least 4 MiB each, so a job's growing frontiers rarely reallocate (a device free synchronises the
+    // whole device, every worker's stream included)
+    const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
+    if((rc = B.vfe.reserve(want)) || (rc = B.ddidx.reserve(std::max<size_t>(n*sizeof(uint32_t), (size_t)1 << 20))) ||
+       (rc = B.ddcnt.reserve(64)) || (rc = H.vfe.reserve(want))) return rc;
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
@@ -157,7 +184,7 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
     JobBufs &B = J.ctx->B; int rc;
     if((rc = B.pts.reserve(n*sizeof(sfx_cell_t)))) return rc;
     hipStream_t s = J.s;
-    SFX_HIP(hipMemcpyAsync(B.pts.p, pts, n*sizeof(sfx_cell_t), hipMemcpyHostToDevice, s));
+    if((rc = upload(*J.ctx, s, B.pts.p, pts, n*sizeof(sfx_cell_t)))) return rc;
     std::vector<uint8_t> fl(n);
     if((rc = eval_nodes_core(J, B.pts.as<sfx_cell_t>(), n, 0, v, fl.data(), nq, u, nd))) return rc;
     for(long i = 0; i < n; i++) flg[i] = (fl[i] & (SFX_FL_NEAR | SFX_FL_DD)) | ((fl[i] & SFX_FL_UNRES) ? SFX_PT_UNRES : 0);
@@ -305,8 +332,8 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         SFX_HIP(level_reduce(s, a));
         if(pon){
             SFX_HIP(hipEventRecord(X.ev[5], s));
-            LevelAcc now_; SFX_HIP(hipMemcpyAsync(&now_, B.acc.p, sizeof now_, hipMemcpyDeviceToHost, s));
-            SFX_HIP(hipStreamSynchronize(s));
+            LevelAcc now_; RB rb = {&now_, B.acc.p, sizeof now_};
+            if((rc = read_back(X, s, &rb, 1))) return rc;
             long dv = (long)now_.nv - prof_nv, dsk = (long)now_.nskip - prof_nsk, dq = (long)now_.nq - prof_nq;
             prof_nv = (long)now_.nv; prof_nsk = (long)now_.nskip; prof_nq = (long)now_.nq;
             prof_add(J.g->name, SFX_K_LEVEL_SPLIT, prof_ms(X, 0, 1), nc, 0, 0);
@@ -318,8 +345,8 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         if(J.o.verbose) fprintf(stderr, "  level %d: cells %ld/%ld\n", lev, c0 + nc, ncur);
         if(ck.ck->due() && c0 + nc < ncur){
             // a checkpoint after this block (sum.c:782-783): the device is drained first
-            LevelAcc a_; SFX_HIP(hipMemcpyAsync(&a_, B.acc.p, sizeof a_, hipMemcpyDeviceToHost, s));
-            SFX_HIP(hipStreamSynchronize(s));
+            LevelAcc a_; RB rb = {&a_, B.acc.p, sizeof a_};
+            if((rc = read_back(X, s, &rb, 1))) return rc;
             if(ck.cur_lev != lev){ if((rc = download_cells(B.cur, ncur, s, ck.cur))) return rc; ck.cur_lev = lev; }
             std::vector<sfx_cell_t> kept;
             if(!last && (rc = download_cells(B.nxt, (long)a_.nkept, s, kept))) return rc;
@@ -339,8 +366,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
             }
         }
     }
-    SFX_HIP(hipMemcpyAsync(out, B.acc.p, sizeof *out, hipMemcpyDeviceToHost, s));
-    SFX_HIP(hipStreamSynchronize(s));
+    { RB rb = {out, B.acc.p, sizeof *out}; if((rc = read_back(X, s, &rb, 1))) return rc; }
     *nnext = (long)out->nkept;
     return SFX_OK;
 }
@@ -393,7 +419,7 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
     std::vector<double> EX(2*K + 1);
     for(int x = -K; x <= K; x++) EX[x + K] = exp(x*h);
     if((rc = J.ctx->B.EX.reserve(EX.size()*sizeof(double)))) return rc;
-    SFX_HIP(hipMemcpyAsync(J.ctx->B.EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
+    if((rc = upload(*J.ctx, J.s, J.ctx->B.EX.p, EX.data(), EX.size()*sizeof(double)))) return rc;
     set_params(J, abserr);
     long q = 0, nd = 0; double u = 0;
     if((rc = eval_nodes_finished(J, pts.data(), n, v, flags, &q, &u, &nd))) return rc;
@@ -429,7 +455,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     std::vector<double> EX(2*J.K + 1);
     for(int k = -J.K; k <= J.K; k++) EX[k + J.K] = exp(k*h);
     if((rc = B.EX.reserve(EX.size()*sizeof(double)))) return rc;
-    SFX_HIP(hipMemcpyAsync(B.EX.p, EX.data(), EX.size()*sizeof(double), hipMemcpyHostToDevice, J.s));
+    if((rc = upload(X, J.s, B.EX.p, EX.data(), EX.size()*sizeof(double)))) return rc;
     set_params(J, J.o.abserr0);
     Ckpt ck; ck_setup(ck, J, N);
     CkState C;
