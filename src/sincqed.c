@@ -5,7 +5,9 @@
  * computed on the GPU through the C ABI of include/sfx.h.
  *
  * As in the reference's GPU build, the symmetry fold is not applied (sum.c:571): SINCQED_SYM is ignored
- * with a note.  SINCQED_GPU=d picks the device (gpu.cu:30-34).
+ * with a note.  SINCQED_GPU=d picks the device (gpu.cu:30-34).  A run of several graphs submits them all
+ * to the library's job engine (SINCQED_WORKERS workers, default 16) and reports them in order as they
+ * finish, with the results and lines of a run one after another; SINCQED_BATCH=0 runs them one at a time.
  */
 #include <math.h>
 #include <stdio.h>
@@ -42,13 +44,12 @@ static int isnum(const char *s){ char *e; strtod(s, &e); return e != s && *e == 
 static double envd(const char *n, double d){ const char *e = getenv(n); return e ? atof(e) : d; }
 static int envi(const char *n, int d){ const char *e = getenv(n); return e ? atoi(e) : d; }
 
-static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opts_t *o, int idx, int ng,
-                     double *tot, double *toterr, double *toterr2){
+/* one graph's log lines and results line (main.c:40-75), from its finished sum; rc the sum's return code */
+static int report_graph(int gi, double h, double eps, int top, const sfx_opts_t *o, int rc, const sfx_result_t *res,
+                        double *tot, double *toterr, double *toterr2){
     sfx_graph_info_t g; sfx_graph_info(gi, &g);
-    say("  graph %d/%d %s (%s, D=%d) started on the GPU\n", idx, ng, g.name, g.volkov, g.D);
-    sfx_result_t R;
-    int rc = sfx_hsum(gi, h, N, eps, top, o, &R);
     if(rc){ fprintf(stderr, "sincqed: %s: %s\n", g.name, sfx_last_error()); return rc; }
+    const sfx_result_t R = *res;
     if(R.resumed)                                          /* sum.c:602 */
         say("# resuming %s from checkpoint: refine to step %dh, cell %ld of %ld\n", g.name, 1 << R.resume_lev,
             R.resume_next, R.resume_ncells);
@@ -87,6 +88,38 @@ static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opt
     }
     *tot += v; *toterr += err; *toterr2 += err*err;
     return 0;
+}
+
+static int run_graph(int gi, double h, int N, double eps, int top, const sfx_opts_t *o, int idx, int ng,
+                     double *tot, double *toterr, double *toterr2){
+    sfx_graph_info_t g; sfx_graph_info(gi, &g);
+    say("  graph %d/%d %s (%s, D=%d) started on the GPU\n", idx, ng, g.name, g.volkov, g.D);
+    sfx_result_t R;
+    int rc = sfx_hsum(gi, h, N, eps, top, o, &R);
+    return report_graph(gi, h, eps, top, o, rc, &R, tot, toterr, toterr2);
+}
+
+/* several graphs at once through the job engine (SINCQED_BATCH, default on): every graph is submitted
+ * first, then the results are reported in the reference's order (main.c:243) as they finish.  Each
+ * result is the one the graph gives alone; the log and results lines are those of one after another. */
+static int run_batched(const int *sel, int ns, double h, int N, double eps, int top, const sfx_opts_t *o, int workers,
+                       double *tot, double *toterr, double *toterr2){
+    int rc = sfx_engine_start(workers, 0);
+    if(rc){ fprintf(stderr, "sincqed: %s\n", sfx_last_error()); return rc; }
+    long *job = (long*)malloc(ns*sizeof(long));
+    for(int j = 0; j < ns; j++)
+        if((rc = sfx_job_submit(sel[j], h, N, eps, top, o, &job[j]))){ fprintf(stderr, "sincqed: %s\n", sfx_last_error()); free(job); return rc; }
+    int first = 0;
+    for(int j = 0; j < ns; j++){
+        sfx_graph_info_t g; sfx_graph_info(sel[j], &g);
+        say("  graph %d/%d %s (%s, D=%d) started on the GPU\n", j + 1, ns, g.name, g.volkov, g.D);
+        sfx_result_t R; double tq, tr;
+        int r = sfx_job_wait(job[j], &R, &tq, &tr);
+        if(!first) first = report_graph(sel[j], h, eps, top, o, r, &R, tot, toterr, toterr2);
+        sfx_job_release(job[j]);
+    }
+    free(job);
+    return first;
 }
 
 /* ---- compare (main.c:118-153): the error of each graph from two results files at different h ---- */
@@ -232,7 +265,10 @@ int main(int argc, char **argv){
       else say("# range |z| <= %g\n", K1*h);
       if(RESF) fprintf(RESF, "# %s  %s\n# window |z| <= %g\n", tb, cmd, K1*h); }
     double tot = 0, toterr = 0, toterr2 = 0;
-    for(int j = 0; j < ns; j++) if((rc = run_graph(sel[j], h, N, eps, top, &o, j + 1, ns, &tot, &toterr, &toterr2))) break;
+    if(ns > 1 && envi("SINCQED_BATCH", 1))
+        rc = run_batched(sel, ns, h, N, eps, top, &o, envi("SINCQED_WORKERS", 16), &tot, &toterr, &toterr2);
+    else
+        for(int j = 0; j < ns; j++) if((rc = run_graph(sel[j], h, N, eps, top, &o, j + 1, ns, &tot, &toterr, &toterr2))) break;
     if(ns > 1 && !rc){
         say("# sum of %d graphs: %+.12f  err %.2e (linear sum)  %.2e (in quadrature)\n", ns, tot, toterr, sqrt(toterr2));
         if(RESF) fprintf(RESF, "# SUM %d graphs  %+.15e  err %.3e (linear)  %.3e (quadrature)\n", ns, tot, toterr, sqrt(toterr2));

@@ -2,6 +2,7 @@
 results-file format (driver/main.c:66-73, 226, 246), each line matching the reference's own output for
 config 1 (tests/golden/config1_sym0.txt, the reference run with SINCQED_SYM=0)."""
 import os
+import re
 import subprocess
 
 import pytest
@@ -72,3 +73,28 @@ def test_run_box_scan_env(tmp_path, oracle):
         _, _, a = _split(mine[g])
         assert int(a[14]) == o['nev'], g
         assert int(a[14]) != oracle.hsum(g, 1.5, 33, 1e-10, 1)['nev'], g
+
+
+@pytest.mark.gpu
+def test_run_batched_equals_one_at_a_time(tmp_path):
+    """several graphs go through the job engine (the default) or one after another (SINCQED_BATCH=0): the
+    same results lines apart from the time column, the same log lines in the same order"""
+    out = {}
+    for mode in ('1', '0'):
+        d = tmp_path / ('b' + mode)
+        d.mkdir()
+        env = dict(os.environ, SINCQED_RESULTS='r.txt', SINCQED_LOG='-', SINCQED_CKPT='0', SINCQED_BATCH=mode)
+        p = subprocess.run([CLI, 'run', '1.0', '97', '1e-12', 'all2', 'v3_00', 'g3_07'], env=env, cwd=str(d),
+                           capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0, p.stderr
+        rec, tail = _lines(d / 'r.txt')
+        lines = [l for l in p.stdout.splitlines() if not l.startswith('# sincqed ')]
+        out[mode] = (rec, tail, lines)
+    (r1, t1, l1), (r0, t0, l0) = out['1'], out['0']
+    assert sorted(r1) == sorted(r0) and len(r1) == 7
+    for g in r0:
+        a, b = _split(r1[g])[2], _split(r0[g])[2]
+        assert a[:15] == b[:15] and a[16:] == b[16:], g                       # all but the time column
+    assert t1 == t0
+    strip = lambda ls: [re.sub(r'\d+\.\ds(\s|$)', 'Ts ', l) for l in ls]      # the per-graph time field
+    assert strip(l1) == strip(l0)
