@@ -80,6 +80,8 @@ typedef struct {
                              (sum.c:166-191, 610-617): every node at step 2^(top+1) h gives the reference
                              scale, every node at step 2^top h is summed and those with |f w| >=
                              1e-3 eps Ref are kept; no shadow tiers (esok = 0).  (2K/2^top + 1)^D nodes. */
+    int abcheck;          /* SINCQED_ABCHECK: the points that may skip the |.| sums are evaluated and tested in
+                             full anyway, and those that needed the fallback are counted (abmiss, eval.c:11-12) */
 } sfx_opts_t;
 
 typedef struct {
@@ -103,6 +105,7 @@ typedef struct {
     int resume_lev;       /*   resume_ncells (sum.c:602) */
     long resume_next, resume_ncells;
     int ckpts_written;    /* checkpoints this call wrote */
+    long abmiss;          /* with opts->abcheck: skip-class points that needed the fallback (ABMISS) */
 } sfx_result_t;
 
 /* ---- library ---- */

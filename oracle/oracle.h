@@ -26,6 +26,7 @@ typedef struct {
     int abskip, dropunres, keepbt, btmargin;
     long maxpts;          /* nodes per refinement block: the reference CPU build's 2^18 (sum.c:643) */
     int grow;             /* SINCQED_GROW: 1 the support-following grow, 0 full-box scans (sum.c:610-617) */
+    int abcheck;          /* SINCQED_ABCHECK (eval.c:11-12) */
 } orc_opts_t;
 
 typedef struct {
@@ -36,6 +37,7 @@ typedef struct {
     long nev_level[16];
     int zero, K, top;
     double abserr;
+    long abmiss;          /* ABMISS: with abcheck, skip-class points that needed the fallback */
 } orc_res_t;
 
 typedef struct { char volkov[64]; int L, D, weight, has_fv; } orc_graph_t;

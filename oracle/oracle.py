@@ -23,7 +23,7 @@ SINCQED_REF_NC = os.path.join(REF_DIR, 'sincqed_ref_nc')
 class OrcOpts(C.Structure):
     _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
-                ('maxpts', C.c_long), ('grow', C.c_int)]
+                ('maxpts', C.c_long), ('grow', C.c_int), ('abcheck', C.c_int)]
 
 
 class OrcRes(C.Structure):
@@ -31,7 +31,7 @@ class OrcRes(C.Structure):
                 ('trunc', C.c_double), ('unres', C.c_double), ('nev', C.c_long), ('nq', C.c_long),
                 ('kept_top', C.c_long), ('es1', C.c_double), ('es2', C.c_double), ('abskipped', C.c_long),
                 ('unresdrop', C.c_long), ('btuns', C.c_long), ('nev_level', C.c_long * 16), ('zero', C.c_int),
-                ('K', C.c_int), ('top', C.c_int), ('abserr', C.c_double)]
+                ('K', C.c_int), ('top', C.c_int), ('abserr', C.c_double), ('abmiss', C.c_long)]
 
 
 class OrcGraph(C.Structure):

@@ -51,8 +51,8 @@ static void run_graph(const graph_t *g, double h, int N, double eps, int top, do
         if(r) fprintf(RESF, " %+.15e %+.3e", r->val, v - r->val); else fprintf(RESF, " %22s %10s", "-", "-");
         fprintf(RESF, "\n"); fflush(RESF);
     }
-    printf("#R %s S0 %.17g S1 %.17g S2 %.17g Ah %.17g ref %.17g prune %.17g trunc %.17g unres %.17g nev %ld nq %ld kept %ld skipped %ld unresdrop %ld t %.3f\n",
-           g->name, R.S[0], R.S[1], R.S[2], R.Ah, R.ref, R.prune, R.trunc, R.unres, R.nev, R.nq, R.kept_top, ABSKIPPED, UNRESDROP, R.t);
+    printf("#R %s S0 %.17g S1 %.17g S2 %.17g Ah %.17g ref %.17g prune %.17g trunc %.17g unres %.17g nev %ld nq %ld kept %ld skipped %ld unresdrop %ld abmiss %ld t %.3f\n",
+           g->name, R.S[0], R.S[1], R.S[2], R.Ah, R.ref, R.prune, R.trunc, R.unres, R.nev, R.nq, R.kept_top, ABSKIPPED, UNRESDROP, ABMISS, R.t);
     fflush(stdout);
     *tot += v; *toterr += err; *toterr2 += err*err;
 }

@@ -24,7 +24,7 @@ const char *orc_error(void){ return ERR; }
 
 void orc_opts_default(orc_opts_t *o){
     o->abserr0 = 1e-15; o->abrel = 1e-3; o->nearf = 1e-3; o->ddslack = 1;   /* eval.c:7,17-18 */
-    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18; o->grow = 1;
+    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18; o->grow = 1; o->abcheck = 0;
 }
 
 /* prefix "G_": the kernels as the reference builds them (libsqk_ref.so); "Q_": built without FP
@@ -135,7 +135,7 @@ int orc_scale(const char *name, double kap, unsigned seed, int E, orc_scale_t *o
 
 /* ---- state of one run ---- */
 static const graph_t *G; static int D, K; static double H, ABSERR; static double *EX; static orc_opts_t O;
-static long NQ, UNRESDROP; static double U;
+static long NQ, UNRESDROP, ABMISS; static double U;
 
 /* lattice.c:30-34 */
 static double lattice_node(const int *k, double *sf){
@@ -151,17 +151,18 @@ static double qvalue(double vq, double abq, double w){
     if(!(e <= fabs(v))){ UNRESDROP++; if(O.dropunres) return 0; }
     return v;
 }
-/* eval.c:33-60: leafx = lattice_node, fd with SQ_AB = !skip, finish */
+/* eval.c:33-60: leafx = lattice_node, fd with SQ_AB = !(skip && !ABCHECK), finish */
 static double leafx(const int *k, int skip, int *near){
     double sf[MAXD], ab = 0; double w = lattice_node(k, sf);
-    if(SETAB) SETAB(!skip);
+    if(SETAB) SETAB(!(skip && !O.abcheck));
     double v = ((fd_t)G->fd)(sf, &ab);
     if(SETAB) SETAB(1);
     *near = 0;
-    if(skip){ v *= w; return isfinite(v) ? v : 0; }
+    if(skip && !O.abcheck){ v *= w; return isfinite(v) ? v : 0; }
     double bnd = ab*w*16*DBL_EPSILON/O.ddslack;
     if(!(bnd <= O.nearf*ABSERR)) *near = 1;
     if(!(bnd <= ABSERR)){
+        if(skip) ABMISS++;
         ddv q[MAXD], abq; for(int i = 0; i < D; i++){ q[i].hi = sf[i]; q[i].lo = 0; }
         ddv vq = ((fq_t)G->fq)(q, &abq); NQ++;
         return qvalue(vq.hi + vq.lo, abq.hi + abq.lo, w);
@@ -380,7 +381,7 @@ int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_o
     free(EX); EX = (double*)malloc((2*K + 1)*sizeof(double));
     for(int k = -K; k <= K; k++) EX[k + K] = exp(k*H);
     memset(R, 0, sizeof *R); R->K = K; R->top = top;
-    NQ = 0; UNRESDROP = 0; U = 0;
+    NQ = 0; UNRESDROP = 0; U = 0; ABMISS = 0;
     ABSERR = O.abserr0;
     if(box_zero(top, &R->nev)){ R->zero = 1; return 0; }
     vec_t cur = {0}; double S, A, e1, e2, dr;
@@ -444,7 +445,7 @@ int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_o
         free(cur.p); cur = nxt;
     }
     free(cur.p);
-    R->nq = NQ; R->unresdrop = UNRESDROP;
+    R->nq = NQ; R->unresdrop = UNRESDROP; R->abmiss = ABMISS;
     R->S[0] = Slev[nl - 1]; R->S[1] = nl > 1 ? Slev[nl - 2] : NAN; R->S[2] = nl > 2 ? Slev[nl - 3] : NAN;
     R->es1 = ES1; R->es2 = ES2;
     return 0;

@@ -23,6 +23,7 @@ typedef struct {
     double nearf;            /* NEARF (eval.c:18) */
     double ddslack;          /* DDSLACK (eval.c:17) */
     int dropunres;           /* DROPUNRES (eval.c:24) */
+    int abcheck;             /* ABCHECK: skip-class points evaluated and tested in full (eval.c:11-12, 35) */
 } sfx_eval_params_t;
 
 /* Per-child flag byte written by the evaluation kernels. */

@@ -39,11 +39,11 @@ struct DevBuf {
 struct ddacc { double hi, lo; };
 
 // per-tile partial of a refinement block (see k_level_partials)
-struct LevelPart { double s[2], a[2], b1[2], b2[2], drop[2], u[2]; uint32_t nv, nskip, nkeep, nq, nd, pad; };
+struct LevelPart { double s[2], a[2], b1[2], b2[2], drop[2], u[2]; uint32_t nv, nskip, nkeep, nq, nd, nmiss; };
 // level accumulator in device memory, added to by every block of the level in block order
 struct LevelAcc {
     double s[2], a[2], b1[2], b2[2], drop[2], u[2];
-    unsigned long long nv, nskip, nkept, nq, nd;
+    unsigned long long nv, nskip, nkept, nq, nd, nmiss;
 };
 
 constexpr int TILE_THREADS = 256;

@@ -221,7 +221,7 @@ static int box_zero(Job &J, int lev, int *zero, long *nev, long *unresdrop){
 
 static void set_params(Job &J, double abserr){
     J.P.K = J.K; J.P.H = J.h; J.P.abskip = J.o.abskip; J.P.abserr = abserr; J.P.nearf = J.o.nearf;
-    J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.EX = J.ctx->B.EX.as<double>();
+    J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.abcheck = J.o.abcheck; J.P.EX = J.ctx->B.EX.as<double>();
 }
 
 // grow a device buffer keeping its first `used` bytes (DevBuf::reserve does not preserve contents)
@@ -316,7 +316,8 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
         hipLaunchKernelGGL(k_cell_split, dim3((unsigned)((nc + 256*SPLIT_ITEMS - 1)/(256*SPLIT_ITEMS))), dim3(256), 0, s, cells, nc, J.o.abskip,
                            B.lsk.as<uint32_t>(), B.lfu.as<uint32_t>(), lc);
         if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
-        if(J.o.abskip) SFX_HIP(J.g->eval(s, MODE_LIST, 0, cells, B.lsk.as<uint32_t>(), lc, n, st, &J.P, B.v.as<double>(),
+        // the skip class with the |.| sums compiled out; with ABCHECK evaluated and tested in full
+        if(J.o.abskip) SFX_HIP(J.g->eval(s, MODE_LIST, J.o.abcheck ? 1 : 0, cells, B.lsk.as<uint32_t>(), lc, n, st, &J.P, B.v.as<double>(),
                                          B.fl.as<uint8_t>(), B.ddidx.as<uint32_t>(), lc + 2));
         if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
         SFX_HIP(J.g->eval(s, MODE_LIST, 1, cells, B.lfu.as<uint32_t>(), lc + 1, n, st, &J.P, B.v.as<double>(),
@@ -357,7 +358,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
             ck_fill_run(c, ck.res, ck.Slev, ck.nl, now() - ck.t0);
             cells_to_pts(ck.cur, D, c.cur); cells_to_pts(kept, D, c.nxt);
             c.es[0] = a_.b1[0] + a_.b1[1]; c.es[1] = a_.b2[0] + a_.b2[1];
-            c.cnt[1] = ck.res->abskipped + (long)a_.nskip;
+            c.cnt[1] = ck.res->abskipped + (long)a_.nskip; c.cnt[2] = ck.res->abmiss + (long)a_.nmiss;
             ck_write(*ck.ck, c);
             ck.res->ckpts_written = ck.ck->written;
             if(ck.ck->stop_after && ck.ck->written >= ck.ck->stop_after){
@@ -470,7 +471,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         t0 -= C.R.t;
         res->resumed = 1; res->resume_lev = C.lev; res->resume_next = C.next; res->resume_ncells = (long)C.cur.size();
         res->ref = ref = C.R.ref; res->prune = C.R.prune; res->trunc = C.R.trunc; res->unres = C.R.unres;
-        res->nev = C.R.nev; res->nq = C.R.nq; res->kept_top = C.R.kept_top; res->abskipped = C.cnt[1];
+        res->nev = C.R.nev; res->nq = C.R.nq; res->kept_top = C.R.kept_top; res->abskipped = C.cnt[1]; res->abmiss = C.cnt[2];
         memcpy(Slev, C.Slev, sizeof(double)*C.nl); nl = C.nl;
         std::vector<sfx_cell_t> cur; pts_to_cells(C.cur, J.D, cur);
         ncur = (long)cur.size();
@@ -527,7 +528,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         double S = A.s[0] + A.s[1];
         res->prune += A.drop[0] + A.drop[1];
         res->nev += (long)A.nv; res->nq += (long)A.nq; res->unresdrop += (long)A.nd;
-        res->abskipped += (long)A.nskip;
+        res->abskipped += (long)A.nskip; res->abmiss += (long)A.nmiss;
         res->nev_level[lev] = (long)A.nv;
         Slev[nl++] = S;
         if(lev == 0){
@@ -543,7 +544,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
                 if((rc = download_cells(B.cur, ncur, J.s, ckr.cur))) { budget_release(held); return rc; }
                 ckr.cur_lev = lev - 1;
                 cells_to_pts(ckr.cur, J.D, c.cur);
-                c.cnt[1] = res->abskipped;
+                c.cnt[1] = res->abskipped; c.cnt[2] = res->abmiss;
                 ck_write(ck, c);
                 res->ckpts_written = ck.written;
                 if(ck.stop_after && ck.written >= ck.stop_after){

@@ -99,7 +99,9 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
             valid = point_coords<D, true>(pts, o, st, P.K, k, &cfl);
         } else valid = point_coords<D, false>(pts, i, st, P.K, k, &cfl);
     }
-    bool skip = valid && P.abskip && (cfl & 1u);
+    // the skip class (ABSKIP and the parent's skip bit); with ABCHECK its points are evaluated and tested in
+    // full (eval.c:35, 57) and the launch for that class uses AB = 1
+    bool skip = valid && P.abskip && (cfl & 1u) && !P.abcheck;
     double v = 0; unsigned fl = 0;
     if(valid){
         double sf[D]; double w = lattice_node<D>(sEX, P.K, P.H, k, sf);

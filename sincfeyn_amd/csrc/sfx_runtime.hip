@@ -133,7 +133,7 @@ __device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, unsigne
         dacc_addd(t.drop[0], t.drop[1], shfl_down(t.drop[0], d), shfl_down(t.drop[1], d));
         dacc_addd(t.u[0], t.u[1], shfl_down(t.u[0], d), shfl_down(t.u[1], d));
         t.nv += __shfl_down(t.nv, d, 64); t.nskip += __shfl_down(t.nskip, d, 64); t.nkeep += __shfl_down(t.nkeep, d, 64);
-        t.nq += __shfl_down(t.nq, d, 64); t.nd += __shfl_down(t.nd, d, 64);
+        t.nq += __shfl_down(t.nq, d, 64); t.nd += __shfl_down(t.nd, d, 64); t.nmiss += __shfl_down(t.nmiss, d, 64);
     }
     __shared__ LevelPart sh[TILE_THREADS/64];
     if(__lane_id() == 0) sh[threadIdx.x >> 6] = t;
@@ -145,7 +145,7 @@ __device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, unsigne
             dacc_addd(r.s[0], r.s[1], q.s[0], q.s[1]); dacc_addd(r.a[0], r.a[1], q.a[0], q.a[1]);
             dacc_addd(r.b1[0], r.b1[1], q.b1[0], q.b1[1]); dacc_addd(r.b2[0], r.b2[1], q.b2[0], q.b2[1]);
             dacc_addd(r.drop[0], r.drop[1], q.drop[0], q.drop[1]); dacc_addd(r.u[0], r.u[1], q.u[0], q.u[1]);
-            r.nv += q.nv; r.nskip += q.nskip; r.nkeep += q.nkeep; r.nq += q.nq; r.nd += q.nd;
+            r.nv += q.nv; r.nskip += q.nskip; r.nkeep += q.nkeep; r.nq += q.nq; r.nd += q.nd; r.nmiss += q.nmiss;
         }
         part[blockIdx.x] = r;
         keepoff[blockIdx.x] = r.nkeep;
@@ -162,8 +162,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
         if(!(f & SFX_FL_VALID)) continue;
         double v = a.v[i], x = v*a.pw, av = fabs(v);
         unsigned cfl = a.cells[i >> a.D].fl;
-        t.nv++; t.nskip += (a.abskip && (cfl & 1u)) ? 1u : 0u;
-        if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
+        const bool sk = a.abskip && (cfl & 1u);
+        t.nv++; t.nskip += sk ? 1u : 0u;
+        if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; t.nmiss += sk ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
         dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
         if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
         else if(av >= a.thr) t.nkeep++;
@@ -177,16 +178,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
 constexpr int CB = 256;
 __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
     __shared__ double sh[6][2][CB];
-    __shared__ unsigned long long shc[5][CB];
+    __shared__ unsigned long long shc[6][CB];
     long per = (nb + CB - 1)/CB, b0 = threadIdx.x*per, b1 = b0 + per < nb ? b0 + per : nb;
     double v[6][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    unsigned long long nv = 0, nsk = 0, nk = 0, nd = 0, nq = 0;
+    unsigned long long nv = 0, nsk = 0, nk = 0, nd = 0, nq = 0, nm = 0;
     for(long b = b0; b < b1; b++){
         const LevelPart &p = a.part[b];
         dacc_addd(v[0][0], v[0][1], p.s[0], p.s[1]); dacc_addd(v[1][0], v[1][1], p.a[0], p.a[1]);
         dacc_addd(v[2][0], v[2][1], p.b1[0], p.b1[1]); dacc_addd(v[3][0], v[3][1], p.b2[0], p.b2[1]);
         dacc_addd(v[4][0], v[4][1], p.drop[0], p.drop[1]); dacc_addd(v[5][0], v[5][1], p.u[0], p.u[1]);
-        nv += p.nv; nsk += p.nskip; nk += p.nkeep; nq += p.nq; nd += p.nd;
+        nv += p.nv; nsk += p.nskip; nk += p.nkeep; nq += p.nq; nd += p.nd; nm += p.nmiss;
     }
     // exclusive scan of the kept counts over the threads' tile ranges, offsets from acc->nkept
     shc[0][threadIdx.x] = nk; __syncthreads();
@@ -201,6 +202,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
     __syncthreads();
     for(int k = 0; k < 6; k++){ sh[k][0][threadIdx.x] = v[k][0]; sh[k][1][threadIdx.x] = v[k][1]; }
     shc[0][threadIdx.x] = nv; shc[1][threadIdx.x] = nsk; shc[2][threadIdx.x] = nk; shc[3][threadIdx.x] = nd; shc[4][threadIdx.x] = nq;
+    shc[5][threadIdx.x] = nm;
     __syncthreads();
     // fixed pairwise tree over the threads
     for(int d = CB/2; d > 0; d >>= 1){
@@ -211,7 +213,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
                 dacc_addd(h, l, sh[k][0][o], sh[k][1][o]);
                 sh[k][0][threadIdx.x] = h; sh[k][1][threadIdx.x] = l;
             }
-            for(int k = 0; k < 5; k++) shc[k][threadIdx.x] += shc[k][o];
+            for(int k = 0; k < 6; k++) shc[k][threadIdx.x] += shc[k][o];
         }
         __syncthreads();
     }
@@ -224,6 +226,7 @@ __global__ void __launch_bounds__(CB) k_level_combine(LevelArgs a, long nb){
         dacc_addd(A->drop[0], A->drop[1], sh[4][0][0], sh[4][1][0]);
         dacc_addd(A->u[0], A->u[1], sh[5][0][0], sh[5][1][0]);
         A->nv += shc[0][0]; A->nskip += shc[1][0]; A->nkept += shc[2][0]; A->nd += shc[3][0]; A->nq += shc[4][0];
+        A->nmiss += shc[5][0];
     }
 }
 

@@ -68,6 +68,7 @@ static int report_graph(int gi, double h, double eps, int top, const sfx_opts_t 
     say("# %s: fallback when eps*sum|terms| > %.1e per point (max of G2_ABSERR and SINCQED_ABREL*eps*Ref; SINCQED_DDSLACK %g)\n",
         g.name, R.abserr, o->ddslack);
     if(o->abskip) say("# %s: %ld of %ld evaluations skipped the |.| sums (no fallback check)\n", g.name, R.abskipped, R.nev);
+    if(o->abcheck) say("# %s: SINCQED_ABCHECK: %ld of the skipped points would have needed the fallback\n", g.name, R.abmiss);   /* main.c:65 */
     if(R.unresdrop)
         say("# WARNING %s: %ld points unresolved even in double-double (eps*sum|terms| > |value|, not one digit left) %s; their bound is in the round column\n",
             g.name, R.unresdrop, o->dropunres ? "counted as 0 (SINCQED_DROPUNRES=0 keeps them)" : "kept as they came out");
@@ -199,6 +200,7 @@ int main(int argc, char **argv){
     o.ddslack = envd("SINCQED_DDSLACK", o.ddslack); o.dropunres = envi("SINCQED_DROPUNRES", o.dropunres);
     o.keepbt = getenv("SINCQED_KEEPBT") != NULL; o.btmargin = envi("SINCQED_BTMARGIN", o.btmargin);
     o.box_scan = envi("SINCQED_GROW", 1) == 0;            /* main.c:183: full-box scans instead of the grow */
+    o.abcheck = envi("SINCQED_ABCHECK", 0);                 /* main.c:179 */
     o.ckpt_every = envd("SINCQED_CKPT", 600);              /* checkpoint.c:9, main.c:184 */
     { const char *d = getenv("SINCQED_CKPT_DIR"); if(d) snprintf(o.ckpt_dir, sizeof o.ckpt_dir, "%s", d); }
     int ng = sfx_graph_count();

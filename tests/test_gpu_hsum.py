@@ -263,3 +263,16 @@ def test_config2_full_size_vs_reference_nc(sfx):
         w = sfx.graphs()[sfx.find(g)]['weight']
         d1, d2 = w*(r['S'][0] - r['es1']), w*(r['es1'] - r['es2'])
         assert 'eps*10 would change the value by %+.3e, eps*100 by a further %+.3e' % (-d1, -d2) in lines[g], (g, lines[g])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps', [('g2_01', 1.0, 97, 1e-12), ('g2_03', 1.0, 97, 1e-12), ('v2_00', 1.0, 97, 1e-12),
+                                          ('g3_07', 1.5, 33, 1e-12)])
+def test_abcheck(sfx, oracle, name, h, N, eps):
+    """SINCQED_ABCHECK (eval.c:11-12, 35): the skip class evaluated and tested in full, its fallback points
+    counted (ABMISS) -- with SINCQED_NEARF=1, where the count is nonzero (the oracle is pinned on the
+    reference there); the default run's values are not changed by the check on these cases"""
+    r, o = _cmp(sfx, oracle, name, h, N, eps, abcheck=1, nearf=1.0)
+    assert r['abmiss'] == o['abmiss'], (r['abmiss'], o['abmiss'])
+    if name in ('g2_03', 'g3_07'):
+        assert r['abmiss'] > 0

@@ -185,3 +185,22 @@ def test_scale_oracle_invariants(g):
 
 def test_scale_oracle_vp_graph_has_none():
     assert O.Oracle(nc=True).scale('v3_00') is None
+
+
+@needs_ref
+@pytest.mark.parametrize('variant', ['ref', 'nc'])
+@pytest.mark.parametrize('h,N,eps,graphs', [(1.0, 97, 1e-12, ['g2_01', 'g2_03', 'v2_00']), (1.5, 33, 1e-12, ['g3_07'])])
+def test_oracle_abcheck_vs_reference(orc, variant, h, N, eps, graphs):
+    """SINCQED_ABCHECK=1 (eval.c:11-12, 35): the skip-class points are evaluated and tested in full and those
+    that needed the fallback counted (ABMISS).  With SINCQED_NEARF=1 (a skip bit for every point not over
+    ABSERR itself) the count is nonzero; the restatement gives the reference's values and counts exactly"""
+    top = O.default_top(h)
+    ref, wall, _ = O.run_reference(h, N, eps, graphs, env={'SINCQED_SYM': '0', 'SINCQED_ABCHECK': '1', 'SINCQED_NEARF': '1'},
+                                   threads=4, nc=(variant == 'nc'))
+    for g in graphs:
+        r = orc[variant].hsum(g, h, N, eps, top, abcheck=1, nearf=1.0)
+        R = ref[g]
+        assert (r['nev'], r['nq'], r['kept_top'], r['abskipped'], r['abmiss']) == \
+               (R['nev'], R['nq'], R['kept'], R['skipped'], R['abmiss']), g
+        assert r['S'][0] == R['S0'] and r['S'][1] == R['S1'], g
+    assert sum(ref[g]['abmiss'] for g in graphs) > 0
