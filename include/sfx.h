@@ -82,6 +82,8 @@ typedef struct {
                              1e-3 eps Ref are kept; no shadow tiers (esok = 0).  (2K/2^top + 1)^D nodes. */
     int abcheck;          /* SINCQED_ABCHECK: the points that may skip the |.| sums are evaluated and tested in
                              full anyway, and those that needed the fallback are counted (abmiss, eval.c:11-12) */
+    int growskip;         /* SINCQED_GROWSKIP (default 2): at top = 0, a frontier point of the grow skips the |.|
+                             sums for up to this many generations after a checked point (sumkey.h:165-168) */
 } sfx_opts_t;
 
 typedef struct {
@@ -106,6 +108,8 @@ typedef struct {
     long resume_next, resume_ncells;
     int ckpts_written;    /* checkpoints this call wrote */
     long abmiss;          /* with opts->abcheck: skip-class points that needed the fallback (ABMISS) */
+    long shell_n;         /* the grow's shell: evaluated points below the running threshold, never expanded; */
+    double shell_sum, shell_abs;   /* their signed sum, and the grow's dropped |.| weight (sum.c:619-626) */
 } sfx_result_t;
 
 /* ---- library ---- */
@@ -173,7 +177,7 @@ int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, cons
 
 /* ---- the hierarchical sum ----
  * hsum(g, h, N, eps, top) for one graph on the device: K = (N-1)/2 rounded up to a multiple of
- * 2^(top+1) (lattice.c:16; K <= 127), top >= 1.  The coarsest level (the support-following grow, or
+ * 2^(top+1) (lattice.c:16; K <= 127), 0 <= top <= 12.  The coarsest level (the support-following grow, or
  * the full-box scans of opts->box_scan, and the dilation) runs on the device with its tables in HBM, only
  * the grow's order-dependent frontier sums on the host; each refinement level runs on the device with
  * the cell lists resident in HBM. */

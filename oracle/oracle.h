@@ -27,6 +27,7 @@ typedef struct {
     long maxpts;          /* nodes per refinement block: the reference CPU build's 2^18 (sum.c:643) */
     int grow;             /* SINCQED_GROW: 1 the support-following grow, 0 full-box scans (sum.c:610-617) */
     int abcheck;          /* SINCQED_ABCHECK (eval.c:11-12) */
+    int growskip;         /* SINCQED_GROWSKIP: skip generations of the grow at top = 0 (sum.c:272-275), default 2 */
 } orc_opts_t;
 
 typedef struct {

@@ -23,7 +23,8 @@ SINCQED_REF_NC = os.path.join(REF_DIR, 'sincqed_ref_nc')
 class OrcOpts(C.Structure):
     _fields_ = [('abserr0', C.c_double), ('abrel', C.c_double), ('nearf', C.c_double), ('ddslack', C.c_double),
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
-                ('maxpts', C.c_long), ('grow', C.c_int), ('abcheck', C.c_int)]
+                ('maxpts', C.c_long), ('grow', C.c_int), ('abcheck', C.c_int),
+                ('growskip', C.c_int)]
 
 
 class OrcRes(C.Structure):

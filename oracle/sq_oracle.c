@@ -24,7 +24,7 @@ const char *orc_error(void){ return ERR; }
 
 void orc_opts_default(orc_opts_t *o){
     o->abserr0 = 1e-15; o->abrel = 1e-3; o->nearf = 1e-3; o->ddslack = 1;   /* eval.c:7,17-18 */
-    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18; o->grow = 1; o->abcheck = 0;
+    o->abskip = 1; o->dropunres = 1; o->keepbt = 0; o->btmargin = 2; o->maxpts = 1L << 18; o->grow = 1; o->abcheck = 0; o->growskip = 2;
 }
 
 /* prefix "G_": the kernels as the reference builds them (libsqk_ref.so); "Q_": built without FP
@@ -207,9 +207,15 @@ static int keyradix(int st){
 }
 static uint64_t key(const pt_t *q){ uint64_t y = 1; for(int i = 0; i < D; i++) y = y*W_ + (uint64_t)(q->k[i]/ST + K/ST); return y; }
 
-/* sumkey.h:130-461 (top >= 1: no skip generations, every point checked) */
-static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, double *e1, double *e2, double *drop, long *nev, long *btuns){
-    int st = 1 << lev;
+/* sumkey.h:130-461.  At top = 0 (gs > 0, sumkey.h:165-168) a frontier point skips the |.| sums when the
+ * point that reached it was checked and not near the fallback, or skipped fewer than gs generations after
+ * such a point (the generation count in PFLAG, sumkey.h:367-368); *gshs, *gshn: the signed sum and count
+ * of the evaluated points below the running threshold (the shell, sum.c:275-276) */
+static long GSKIP;              /* frontier points that skipped the |.| sums (ABSKIPPED) */
+static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, double *e1, double *e2, double *drop, long *nev, long *btuns,
+                double *gshs, long *gshn){
+    int st = 1 << lev, gs = lev == 0 && O.abskip && D < MAXD ? O.growskip : 0;
+    *gshs = 0; *gshn = 0;
     if(!keyradix(st)){ snprintf(ERR, sizeof ERR, "grid keys do not fit"); return -1; }
     double pw = pow(st, D), a = 0, x1 = 0, x2 = 0, dr = 0; ksum_t ks = {0, 0}; long ne = 0;
     vec_t fr = {0}, nx = {0}, pk = {0};
@@ -218,18 +224,23 @@ static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, doub
     hmap_t bt; hinit(&bt, 1 << 16);            /* boundary point -> frontier (all tables; freed ones masked by BTLO) */
     int lay = 0; long BTLO = 0;
     while(fr.n){
-        double *v = (double*)malloc(fr.n*sizeof(double)); unsigned char *nr = (unsigned char*)malloc(fr.n);
-        for(long i = 0; i < fr.n; i++){ int k[MAXD], n_; for(int d = 0; d < D; d++) k[d] = fr.p[i].k[d]; v[i] = leafx(k, 0, &n_); nr[i] = (unsigned char)n_; }
+        double *v = (double*)malloc(fr.n*sizeof(double)); unsigned char *nr = (unsigned char*)malloc(fr.n), *sk = (unsigned char*)malloc(fr.n);
+        for(long i = 0; i < fr.n; i++){
+            int k[MAXD], n_; for(int d = 0; d < D; d++) k[d] = fr.p[i].k[d];
+            sk[i] = gs && fr.p[i].k[PFLAG] > 0; GSKIP += sk[i];
+            v[i] = leafx(k, sk[i], &n_); nr[i] = (unsigned char)n_;
+        }
         ne += fr.n;
         for(long i = 0; i < fr.n; i++){ kadd(&ks, v[i]*pw); a += fabs(v[i])*pw; }
         double thr = eps_rel*a;
-        vec_t kept = {0};
+        vec_t kept = {0}; long *kfi = (long*)malloc((fr.n ? fr.n : 1)*sizeof(long)); long nkf = 0;
         for(long i = 0; i < fr.n; i++){
             int mx = 0; for(int d = 0; d < D; d++){ int x = abs(fr.p[i].k[d]); if(x > mx) mx = x; }
             double w = fabs(v[i])*pw; int kp = fabs(v[i]) >= thr;
             if(mx == K) x1 += w; else if(mx == K - st) x2 += w;
-            if(!kp){ dr += w; *hput(&bt, key(&fr.p[i]), NULL) = lay; }
+            if(!kp){ dr += w; *gshs += v[i]*pw; (*gshn)++; *hput(&bt, key(&fr.p[i]), NULL) = lay; }
             else {
+                kfi[nkf++] = i;
                 pt_t q = fr.p[i]; q.k[PFLAG] = (signed char)(O.abskip ? !nr[i] : 0);
                 vpush(&kept, &fr.p[i]); vpush(keep, &q);
                 if(nkv == kvcap){ kvcap = kvcap ? 2*kvcap : 1024; kv = (double*)realloc(kv, kvcap*sizeof(double)); }
@@ -255,7 +266,9 @@ static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, doub
                 if(hit) continue;
                 int fresh; hput(&hs, y, &fresh);
                 if(!fresh) continue;
-                pt_t c = *p; c.k[d] = (signed char)kk; c.k[PFLAG] = 0; vpush(&nx, &c);
+                pt_t c = *p; c.k[d] = (signed char)kk; c.k[PFLAG] = 0;
+                if(gs){ long pi = kfi[j]; int pg = fr.p[pi].k[PFLAG]; c.k[PFLAG] = (signed char)(!sk[pi] ? !nr[pi] : (pg < gs ? pg + 1 : 0)); }
+                vpush(&nx, &c);
             }
         }
         hfree(&hs);
@@ -266,13 +279,13 @@ static int grow(int lev, double eps_rel, vec_t *keep, double *S, double *A, doub
             mn -= O.btmargin;
             if(mn - 1 > BTLO) BTLO = mn - 1 < lay + 1 ? mn - 1 : lay + 1;
         }
-        free(v); free(nr);
+        free(v); free(nr); free(sk); free(kfi);
         vec_t t = fr; fr = nx; nx = t; lay++;
     }
     double thr = eps_rel*a; long m = 0;                       /* sumkey.h:397-401 */
     for(long i = 0; i < keep->n; i++){ if(kv[i] >= thr){ kv[m] = kv[i]; keep->p[m] = keep->p[i]; m++; } else dr += kv[i]*pw; }
     keep->n = m;
-    if(D < MAXD && m){                                        /* sumkey.h:402-453 shadow tiers */
+    if(D < MAXD && m && lev > 0){                             /* sumkey.h:402-453 shadow tiers (none at top = 0) */
         hmap_t ix; hinit(&ix, m);
         for(long i = 0; i < m; i++) *hput(&ix, key(&keep->p[i]), NULL) = i;
         unsigned char *tr = (unsigned char*)calloc(m, 1); long *cu = (long*)malloc(m*sizeof(long)), *nw = (long*)malloc(m*sizeof(long));
@@ -373,7 +386,7 @@ static int box_zero(int lev, long *nev){
  * each block summed in child order with Neumaier sums, blocks combined by kblock */
 int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_opts_t *o, orc_res_t *R){
     G = findg(name); if(!G) return -1;
-    if(top < 1){ snprintf(ERR, sizeof ERR, "top >= 1 only"); return -1; }
+    if(top < 0){ snprintf(ERR, sizeof ERR, "top >= 0 only"); return -1; }
     orc_opts_t dflt; orc_opts_default(&dflt); O = o ? *o : dflt;
     D = G->D; H = h;
     int m = 1 << (top + 1); K = (N - 1)/2; K = ((K + m - 1)/m)*m;            /* lattice.c:16 */
@@ -381,12 +394,12 @@ int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_o
     free(EX); EX = (double*)malloc((2*K + 1)*sizeof(double));
     for(int k = -K; k <= K; k++) EX[k + K] = exp(k*H);
     memset(R, 0, sizeof *R); R->K = K; R->top = top;
-    NQ = 0; UNRESDROP = 0; U = 0; ABMISS = 0;
+    NQ = 0; UNRESDROP = 0; U = 0; ABMISS = 0; GSKIP = 0;
     ABSERR = O.abserr0;
     if(box_zero(top, &R->nev)){ R->zero = 1; return 0; }
-    vec_t cur = {0}; double S, A, e1, e2, dr;
+    vec_t cur = {0}; double S, A, e1, e2, dr, gshs = 0; long gshn = 0;
     long nev0 = R->nev;
-    if(O.grow){ if(grow(top, eps*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev, &R->btuns)) return -1; R->ref = A; }
+    if(O.grow){ if(grow(top, eps*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev, &R->btuns, &gshs, &gshn)) return -1; R->ref = A; }
     else {
         /* SINCQED_GROW=0 (sum.c:610-617): the reference scale from the box at step 2^(top+1) h, then the
          * box at step 2^top h with the keep threshold (its own A only decides "identically zero" below);
@@ -396,11 +409,14 @@ int orc_hsum(const char *name, double h, int N, double eps, int top, const orc_o
         scan(top, eps*R->ref*1e-3, &cur, &S, &A, &e1, &e2, &dr, &R->nev);
     }
     R->nev_level[top] = R->nev - nev0;
-    R->nq = NQ; R->prune = dr; R->unres = U;
-    if(A == 0){ free(cur.p); R->unresdrop = UNRESDROP; return 0; }
+    R->nq = NQ; R->prune = dr; R->unres = U; R->abskipped = GSKIP;
+    /* top = 0 (sum.c:619-626): every evaluated point is in the sum; the prune column is the signed sum of
+     * the never-expanded shell */
+    if(O.grow && top == 0 && gshn) R->prune = fabs(gshs);
+    if(A == 0){ free(cur.p); R->unresdrop = UNRESDROP; R->abmiss = ABMISS; return 0; }
     { double r = e2 > 0 ? e1/e2 : 0; R->trunc = (r > 0 && r < 1) ? e1*r/(1 - r) : e1; }
     double Slev[16]; int nl = 0; Slev[nl++] = S;
-    if(dilate(&cur, 1 << top)) return -1;
+    if(top > 0 && dilate(&cur, 1 << top)) return -1;
     R->kept_top = cur.n;
     ABSERR = fmax(O.abserr0, O.abrel*eps*R->ref);                            /* sum.c:651 */
     R->abserr = ABSERR;

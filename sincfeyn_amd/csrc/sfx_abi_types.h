@@ -34,6 +34,7 @@ typedef struct {
 #define SFX_FL_UNRES 16u     /* unresolved even in double-double (set by the fallback kernel) */
 #define SFX_FL_SHELL1 32u    /* node lists at step st > 0: max_d |k_d| == K       (the e1 shell, sum.c:178) */
 #define SFX_FL_SHELL2 64u    /* node lists at step st > 0: max_d |k_d| == K - st  (the e2 shell) */
+#define SFX_FL_SKIPCLS 128u  /* node lists: the node is of the skip class (evaluated without the |.| sums unless ABCHECK) */
 
 #ifdef __cplusplus
 }

@@ -198,6 +198,8 @@ struct GrowOut {
     double ref = 0;                 // the reference scale Ref: A of the grow, of the coarser box for the box scans
     // host seconds of the stages (the verbose report, and t_grow / t_dilate of the result)
     double t_loop = 0, t_final = 0, t_tiers = 0, t_dilate = 0; long frontiers = 0, tier_steps = 0;
+    double gshs = 0; long gshn = 0;   // the shell: evaluated points below the running threshold, never expanded
+    long nskip = 0, nmiss = 0;        // frontier points of the skip class, and those that needed the fallback
 };
 
 // box_zero (reference sum.c:346-359): z = 0 and 4096 pseudo-random nodes of the step-2^lev lattice; the
@@ -433,7 +435,7 @@ extern "C" int sfx_eval_finished(int graph, double h, int K, long n, const int16
 
 static int check_job(int graph, double h, int N, double eps, int top){
     if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("bad graph index %d", graph); return SFX_EINVAL; }
-    if(top < 1 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("need h > 0, N >= 3, eps > 0, 1 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
+    if(top < 0 || top > 12 || !(h > 0) || N < 3 || !(eps > 0)){ set_error("need h > 0, N >= 3, eps > 0, 0 <= top <= 12 (got h=%g N=%d eps=%g top=%d)", h, N, eps, top); return SFX_EINVAL; }
     int m = 1 << (top + 1), K = (N - 1)/2; K = ((K + m - 1)/m)*m;
     if(K > 127){ set_error("N = %d gives K = %d; points are stored in 8 bits, so K <= 127 (N <= 255)", N, K); return SFX_EINVAL; }
     return SFX_OK;
@@ -496,8 +498,13 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         if(J.o.box_scan){ if((rc = scan_top(J, top, eps, G, &ncur))) return rc; }
         else { if((rc = grow_device(J, top, eps*1e-3, G, &ncur))) return rc; G.ref = G.A; }
         res->nev += G.nev; res->nq += G.nq; res->unres += G.u; res->unresdrop += G.nd; res->btuns = G.btuns;
+        res->abskipped += G.nskip; res->abmiss += G.nmiss;
         res->nev_level[top] = G.nev;
         res->ref = ref = G.ref; res->prune = G.drop;
+        res->shell_n = G.gshn; res->shell_sum = G.gshs; res->shell_abs = G.drop;
+        // top = 0 (sum.c:619-626): every evaluated point is in the sum; the prune column is the signed sum of
+        // the never-expanded shell, an estimate of what lies beyond it
+        if(!J.o.box_scan && top == 0 && G.gshn) res->prune = fabs(G.gshs);
         double t1 = now(); res->t_grow = t1 - t0 - G.t_dilate; res->t_dilate = G.t_dilate;
         if(J.o.verbose)
             fprintf(stderr, "  %s coarsest level: %.2f ms (%ld frontiers %.2f ms, final threshold %.2f ms, tiers %ld steps %.2f ms, "
@@ -512,7 +519,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     double abserr = fmax(J.o.abserr0, J.o.abrel*eps*ref);          // sum.c:651
     res->abserr = abserr;
     set_params(J, abserr);
-    res->esok = resumed ? C.esok : J.D < 12 && !J.o.box_scan;     // the box scans set no shadow tiers (sum.c:614)
+    res->esok = resumed ? C.esok : J.D < 12 && !J.o.box_scan && top > 0;   // sum.c:597, 614
     CkRun ckr; ckr.ck = &ck; ckr.res = res; ckr.Slev = Slev; ckr.t0 = t0;
     double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
     budget_acquire(held, 0);

@@ -67,8 +67,9 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
     return w;
 }
 
-// MODE_NODES: point i is record i (with st > 0 the flag byte also carries the node's outer-shell class
-// at step st: SFX_FL_SHELL1 for max_d |k_d| = K, SFX_FL_SHELL2 for K - st).  MODE_LIST: the children of the cells of a list: point i is child
+// MODE_NODES: point i is record i, of the skip class when its flag byte is nonzero (SFX_FL_SKIPCLS in
+// the output flags); with st > 0 the flag byte also carries the node's outer-shell class at step st:
+// SFX_FL_SHELL1 for max_d |k_d| = K, SFX_FL_SHELL2 for K - st.  MODE_LIST: the children of the cells of a list: point i is child
 // b = i mod 2^D of cell c = list[i >> D] of pts, written to slot (c << D) | b -- so a block's cells can be
 // run as two launches (cells whose children skip the |.| sums, and the others) into one value array.
 // With ddidx, the slots flagged for the high-precision recompute are appended to ddidx[0 .. *ddcnt), in
@@ -99,9 +100,11 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
             valid = point_coords<D, true>(pts, o, st, P.K, k, &cfl);
         } else valid = point_coords<D, false>(pts, i, st, P.K, k, &cfl);
     }
-    // the skip class (ABSKIP and the parent's skip bit); with ABCHECK its points are evaluated and tested in
+    // the skip class: ABSKIP and the parent's skip bit (cells), or a node's nonzero flag byte (the grow's
+    // skip generations at top = 0, sumkey.h:165-168); with ABCHECK its points are evaluated and tested in
     // full (eval.c:35, 57) and the launch for that class uses AB = 1
-    bool skip = valid && P.abskip && (cfl & 1u) && !P.abcheck;
+    const bool cls = valid && P.abskip && (MODE == MODE_NODES ? cfl != 0u : (cfl & 1u) != 0u);
+    bool skip = cls && !P.abcheck;
     double v = 0; unsigned fl = 0;
     if(valid){
         double sf[D]; double w = lattice_node<D>(sEX, P.K, P.H, k, sf);
@@ -113,7 +116,7 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
             if(__all(skip)) f = G::template f<double>(sf, &ab, 0);
             else f = G::template f<double>(sf, &ab, 1);
         }
-        v = f*w; fl = SFX_FL_VALID;
+        v = f*w; fl = SFX_FL_VALID | (MODE == MODE_NODES && cls ? SFX_FL_SKIPCLS : 0u);
         if(MODE == MODE_NODES && st > 0){
             // the node's outer-shell class at step st (the grow's and the box scan's e1/e2 weights)
             int mx = 0;

@@ -31,7 +31,8 @@ class Opts(C.Structure):
                 ('abskip', C.c_int), ('dropunres', C.c_int), ('keepbt', C.c_int), ('btmargin', C.c_int),
                 ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
                 ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int),
-                ('box_scan', C.c_int), ('abcheck', C.c_int)]
+                ('box_scan', C.c_int), ('abcheck', C.c_int),
+                ('growskip', C.c_int)]
 
 
 class Result(C.Structure):
@@ -42,7 +43,8 @@ class Result(C.Structure):
                 ('abserr', C.c_double), ('K', C.c_int), ('top', C.c_int), ('levels', C.c_int),
                 ('t_grow', C.c_double), ('t_dilate', C.c_double), ('t_refine', C.c_double),
                 ('resumed', C.c_int), ('resume_lev', C.c_int), ('resume_next', C.c_long), ('resume_ncells', C.c_long),
-                ('ckpts_written', C.c_int), ('abmiss', C.c_long)]
+                ('ckpts_written', C.c_int), ('abmiss', C.c_long),
+                ('shell_n', C.c_long), ('shell_sum', C.c_double), ('shell_abs', C.c_double)]
 
 
 class GraphInfo(C.Structure):

@@ -54,6 +54,9 @@ static int report_graph(int gi, double h, double eps, int top, const sfx_opts_t 
         say("# resuming %s from checkpoint: refine to step %dh, cell %ld of %ld\n", g.name, 1 << R.resume_lev,
             R.resume_next, R.resume_ncells);
     if(R.zero) say("# %s: integrand is exactly zero at %d probe nodes; reported as zero without a scan\n", g.name, 4097);
+    if(top == 0 && !o->box_scan && R.shell_n)              /* sum.c:619-626 */
+        say("# %s: grow shell (evaluated and summed, not expanded): %ld points, sum %+.3e, sum|.| %.3e; prune = |shell sum|, an estimate of what lies beyond it\n",
+            g.name, R.shell_n, g.weight*R.shell_sum, g.weight*R.shell_abs);
     double w = g.weight, v = w*R.S[0], d2 = fabs(R.S[1] - R.S[0]);
     double disc = R.Ah > 0 ? 2*d2*d2/R.Ah : 0;
     double err = w*(disc + R.prune + R.trunc + R.unres);
@@ -201,6 +204,7 @@ int main(int argc, char **argv){
     o.keepbt = getenv("SINCQED_KEEPBT") != NULL; o.btmargin = envi("SINCQED_BTMARGIN", o.btmargin);
     o.box_scan = envi("SINCQED_GROW", 1) == 0;            /* main.c:183: full-box scans instead of the grow */
     o.abcheck = envi("SINCQED_ABCHECK", 0);                 /* main.c:179 */
+    o.growskip = envi("SINCQED_GROWSKIP", o.growskip);      /* sum.c:334 */
     o.ckpt_every = envd("SINCQED_CKPT", 600);              /* checkpoint.c:9, main.c:184 */
     { const char *d = getenv("SINCQED_CKPT_DIR"); if(d) snprintf(o.ckpt_dir, sizeof o.ckpt_dir, "%s", d); }
     int ng = sfx_graph_count();

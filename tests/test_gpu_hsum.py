@@ -152,12 +152,13 @@ def test_edge_cases(sfx, oracle, name, h, N, eps, top):
 @pytest.mark.gpu
 def test_bad_arguments_are_refused(sfx):
     """errors are return codes (never exit(), gpu.cu:16): a lattice wider than 8-bit points, top out of
-    range, a graph index out of range"""
+    range (0 .. 12), a graph index out of range"""
     L = sfx.lib()
     o = sfx.opts(); r = sfx.Result()
     assert L.sfx_hsum(sfx.find('g2_00'), 0.1, 1001, 1e-12, 1, sfx.C.byref(o), sfx.C.byref(r)) == -1
     assert b'K <= 127' in L.sfx_last_error()
-    assert L.sfx_hsum(sfx.find('g2_00'), 1.0, 97, 1e-12, 0, sfx.C.byref(o), sfx.C.byref(r)) == -1
+    assert L.sfx_hsum(sfx.find('g2_00'), 1.0, 97, 1e-12, 13, sfx.C.byref(o), sfx.C.byref(r)) == -1
+    assert L.sfx_hsum(sfx.find('g2_00'), 1.0, 97, 1e-12, -1, sfx.C.byref(o), sfx.C.byref(r)) == -1
     assert L.sfx_hsum(9999, 1.0, 97, 1e-12, 1, sfx.C.byref(o), sfx.C.byref(r)) == -1
 
 
@@ -276,3 +277,23 @@ def test_abcheck(sfx, oracle, name, h, N, eps):
     assert r['abmiss'] == o['abmiss'], (r['abmiss'], o['abmiss'])
     if name in ('g2_03', 'g3_07'):
         assert r['abmiss'] > 0
+
+
+TOP0 = [('g2_00', 1.0, 97, 1e-12, {}), ('g2_01', 1.0, 97, 1e-12, {}), ('g2_03', 1.0, 97, 1e-12, {}), ('v2_00', 1.0, 97, 1e-12, {}),
+        ('v3_00', 1.5, 33, 1e-10, {}), ('g3_07', 1.5, 33, 1e-10, {}),
+        ('g2_01', 1.0, 97, 1e-12, dict(growskip=0)), ('g2_03', 1.0, 97, 1e-12, dict(growskip=5, abcheck=1))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps,kw', TOP0)
+def test_top0(sfx, oracle, name, h, N, eps, kw):
+    """top = 0: the grow at step h is the whole sum (no refinement); its frontier points skip the |.| sums
+    for up to growskip generations after a checked point, and the prune column is the shell's signed sum
+    (sum.c:619-626) -- the oracle's counts (evaluations, fallbacks, skipped, ABMISS) and values"""
+    r, o = _cmp(sfx, oracle, name, h, N, eps, 0, **kw)
+    assert r['levels'] == 1 and r['esok'] == 0 and math.isnan(r['S'][1])
+    assert r['abmiss'] == o['abmiss']
+    for k in ('prune', 'trunc', 'unres'):
+        assert abs(r[k] - o[k]) <= 1e-12*abs(o[k]) + 1e-300, (k, r[k], o[k])
+    if not kw:
+        assert r['abskipped'] > 0

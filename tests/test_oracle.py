@@ -204,3 +204,31 @@ def test_oracle_abcheck_vs_reference(orc, variant, h, N, eps, graphs):
                (R['nev'], R['nq'], R['kept'], R['skipped'], R['abmiss']), g
         assert r['S'][0] == R['S0'] and r['S'][1] == R['S1'], g
     assert sum(ref[g]['abmiss'] for g in graphs) > 0
+
+
+@needs_ref
+@pytest.mark.parametrize('variant', ['ref', 'nc'])
+@pytest.mark.parametrize('h,N,eps,graphs,env', [(1.0, 97, 1e-12, ['g2_00', 'g2_01', 'g2_03', 'v2_00'], {}),
+                                                 (1.5, 33, 1e-10, ['v3_00', 'g3_07'], {}),
+                                                 (1.0, 97, 1e-12, ['g2_01', 'g2_03'], {'SINCQED_GROWSKIP': '0'}),
+                                                 (1.0, 97, 1e-12, ['g2_03'], {'SINCQED_GROWSKIP': '5', 'SINCQED_ABCHECK': '1'})])
+def test_oracle_top0_vs_reference(orc, variant, h, N, eps, graphs, env):
+    """top = 0 (sum.c:619-626, sumkey.h:165-168, 367-368): the grow at step h is the whole sum; its frontier
+    points skip the |.| sums for up to SINCQED_GROWSKIP generations after a checked point, and the prune
+    column is the signed sum of the never-expanded shell -- bit for bit against the reference"""
+    e = {'SINCQED_SYM': '0'}
+    e.update(env)
+    ref, wall, _ = O.run_reference(h, N, eps, graphs, top=0, env=e, threads=4, nc=(variant == 'nc'))
+    kw = {}
+    if 'SINCQED_GROWSKIP' in env:
+        kw['growskip'] = int(env['SINCQED_GROWSKIP'])
+    if 'SINCQED_ABCHECK' in env:
+        kw['abcheck'] = 1
+    for g in graphs:
+        r = orc[variant].hsum(g, h, N, eps, 0, **kw)
+        R = ref[g]
+        assert (r['nev'], r['nq'], r['kept_top'], r['abskipped'], r['abmiss']) == \
+               (R['nev'], R['nq'], R['kept'], R['skipped'], R['abmiss']), g
+        assert r['S'][0] == R['S0'] and math.isnan(r['S'][1]) and math.isnan(R['S1']), g
+        for k in ('ref', 'prune', 'trunc', 'unres', 'Ah'):
+            assert r[k] == R[k], (g, k)
