@@ -3,7 +3,8 @@
  * checker; the product library never links it).
  *
  * It restates, single-threaded and in plain C, reasther/sincFeyn v0.9.0:
- *   driver/sum.c:556-814    hsum() with the symmetry fold, predictor, dump and histograms off
+ *   driver/sum.c:556-814    hsum() with the symmetry fold, predictor, dump and histograms off (top >= 0,
+ *                           ABCHECK, the grow's skip generations at top = 0)
  *   driver/sumkey.h:20-461  grow() and dilate() (64-bit grid keys)
  *   driver/sum.c:166-191,610-617 scan(): the full-box scans of SINCQED_GROW=0 (plain sums in node order,
  *                           the reference's OpenMP reduction on one thread)
