@@ -88,5 +88,4 @@ hipError_t scan_sums(hipStream_t s, const ScanArgs &a);
 // order within the block; slots at or past out_cap are not written
 hipError_t scan_keep(hipStream_t s, const ScanArgs &a);
 
-
 }  // namespace sfx

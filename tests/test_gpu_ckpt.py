@@ -102,3 +102,22 @@ def test_device_resumes_reference_checkpoint(sfx, tmp_path):
     b = ref[g]
     assert (r['nev'], r['nq'], r['kept_top']) == (b['nev'], b['nq'], b['kept'])
     assert _close(r['S'][0], b['S0']) and _close(r['S'][1], b['S1'])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('stop', [3, 21])
+def test_box_scan_and_abcheck_stop_and_resume(sfx, tmp_path, stop):
+    """a sum whose coarsest level came from the full-box scans, with the ABCHECK counter running: stopped
+    inside level 1 or inside the last level, it resumes to the unbroken result -- the shadow sums stay off
+    (the file's esok) and the skipped and ABMISS counters continue from the file (cnt[1], cnt[2])"""
+    g, h, N, eps = CASE
+    opt = dict(box_scan=1, abcheck=1, nearf=1.0, block_points=1 << 18)
+    full = sfx.hsum(g, h, N, eps, **opt)
+    kw = dict(opt, ckpt_every=1e-9, ckpt_dir=str(tmp_path))
+    with pytest.raises(sfx.SfxStopped):
+        sfx.hsum(g, h, N, eps, ckpt_stop_after=stop, **kw)
+    r = sfx.hsum(g, h, N, eps, **kw)
+    assert r['resumed'] == 1 and r['esok'] == 0 and full['esok'] == 0
+    assert _same_counts(r, full)
+    assert (r['abskipped'], r['abmiss']) == (full['abskipped'], full['abmiss'])
+    assert _close(r['S'][0], full['S'][0]) and _close(r['S'][1], full['S'][1])

@@ -98,3 +98,20 @@ def test_run_batched_equals_one_at_a_time(tmp_path):
     assert t1 == t0
     strip = lambda ls: [re.sub(r'\d+\.\ds(\s|$)', 'Ts ', l) for l in ls]      # the per-graph time field
     assert strip(l1) == strip(l0)
+
+
+@pytest.mark.gpu
+def test_run_top0(tmp_path, oracle):
+    """`sincqed run 1.0 97 1e-12 0 g2_01`: top = 0 from the command line, the shell line of sum.c:622 in the
+    log, the oracle's evaluation count and value in the results line"""
+    res = tmp_path / 'r.txt'
+    env = dict(os.environ, SINCQED_RESULTS=str(res), SINCQED_LOG='-', SINCQED_CKPT='0')
+    p = subprocess.run([CLI, 'run', '1.0', '97', '1e-12', '0', 'g2_01'], env=env, cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert '# g2_01: grow shell (evaluated and summed, not expanded):' in p.stdout
+    o = oracle.hsum('g2_01', 1.0, 97, 1e-12, 0)
+    _, _, a = _split(_lines(res)[0]['g2_01'])
+    assert a[6] == '0' and int(a[14]) == o['nev']
+    w = oracle.graph('g2_01')['weight']                                   # value = w S_h (main.c:46)
+    assert abs(float(a[7]) - w*o['S'][0]) <= 1e-12*abs(w*o['S'][0])
