@@ -68,6 +68,7 @@ def main(argv=None):
     ap.add_argument('--workers', type=int, default=16)
     ap.add_argument('--dd', default='all', help="all, none, or N:SEED (a seeded subset of N jobs)")
     ap.add_argument('--qd', action='store_true', help='also the all-QD run of the two-loop jobs')
+    ap.add_argument('--progress', action='store_true', help='the all-DD jobs report their refinement blocks on stderr')
     ap.add_argument('--out', default=None)
     a = ap.parse_args(argv)
     info = sfx.init(0)
@@ -101,10 +102,18 @@ def main(argv=None):
             pick.sort()
         print('# all-DD pass: %d jobs' % len(pick), file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        ids = [(i, sfx.submit(*jobs[i], **ALL_DD)) for i in pick]
-        for i, j in ids:
-            dd[i] = sfx.wait(j)[0]
-            print('#   all-DD %s h=%g done' % (G[jobs[i][0]]['name'], jobs[i][1]), file=sys.stderr, flush=True)
+        # the longest of these jobs (the light-by-light integral, every point in double-double) run for
+        # minutes: with --progress they report their refinement blocks, and every job is reported as it
+        # finishes, in whatever order
+        kw = dict(ALL_DD, verbose=1) if a.progress else ALL_DD
+        ids = dict((sfx.submit(*jobs[i], **kw), i) for i in pick)
+        while ids:
+            for j in [j for j in ids if sfx.poll(j) >= 2]:
+                i = ids.pop(j)
+                dd[i] = sfx.wait(j)[0]
+                print('#   all-DD %s h=%g done (%.0f s)' % (G[jobs[i][0]]['name'], jobs[i][1], time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+            time.sleep(0.05)
         t_dd = time.perf_counter() - t0
     qd = {}
     if a.qd:
