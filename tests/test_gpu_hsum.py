@@ -137,6 +137,8 @@ EDGE = [
     ('g2_03', 1.0, 97, 1e-12, 3),      # top = 3: three refinement levels
     ('v3_p4Vc', 1.0, 97, 1e-12, 1),    # identically zero: box_zero reports 0 without a grow
     ('g3_07', 2.0, 25, 1e-10, 2),      # a three-loop graph on a coarse lattice at top = 2
+    ('v2_00', 0.25, 249, 1e-15, 1),    # the widest lattice of 8-bit points at top = 1: K = 124, support out to the edge
+    ('g2_01', 0.5, 241, 1e-12, 2),     # the widest at top = 2: K = 120, support out to the edge
 ]
 
 
@@ -145,6 +147,8 @@ EDGE = [
 def test_edge_cases(sfx, oracle, name, h, N, eps, top):
     """small, coarse, loose, tight, deep and identically-zero sums: the oracle's counts and values"""
     r, o = _cmp(sfx, oracle, name, h, N, eps, top)
+    if N > 200:
+        assert r['K'] >= 120 and o['trunc'] > 0 and abs(r['trunc'] - o['trunc']) <= 1e-12*o['trunc']   # the edge shells are summed
     if name == 'v3_p4Vc':
         assert r['zero'] == 1 and r['nev'] == 4097 and r['S'][0] == 0
 
