@@ -15,6 +15,7 @@ in _gen/ are compiled as they are.
 import argparse
 import concurrent.futures as cf
 import glob
+import json
 import os
 import subprocess
 import sys
@@ -68,12 +69,25 @@ def write_registry(names):
     return path
 
 
+def unit_flags(src):
+    """extra hipcc flags of one translation unit: the integrands emitted with node_split (their FP64 point
+    function as one switch case per forest node) are scheduled bottom-up for register need, which keeps
+    one case within the 256 VGPRs of two waves per SIMD (g3_lbl: 244; the default scheduler's 258 leaves
+    one wave).  Scheduling only: the same instructions on the same operands."""
+    meta = src[:-4] + '.json'
+    if src.endswith('.hip') and os.path.exists(meta):
+        with open(meta) as f:
+            if json.load(f).get('node_split'):
+                return ['-mllvm', '-pre-RA-sched=list-burr']
+    return []
+
+
 def compile_one(src, obj, deps, force):
     if not force and os.path.exists(obj):
         t = os.path.getmtime(obj)
         if all(os.path.getmtime(d) <= t for d in [src] + deps):
             return obj, 0.0, ''
-    cmd = [HIPCC] + HIPFLAGS + ['-c', src, '-o', obj + '.tmp']
+    cmd = [HIPCC] + HIPFLAGS + unit_flags(src) + ['-c', src, '-o', obj + '.tmp']
     import time
     t0 = time.time()
     p = subprocess.run(cmd, capture_output=True, text=True)

@@ -31,6 +31,15 @@
 #ifndef SFX_CHUNK_INLINE
 #define SFX_CHUNK_INLINE(R) (sizeof(R) == sizeof(double))
 #endif
+// Point functions whose forest nodes are all leaves are emitted a second time for the FP64 kernels as a
+// loop over the nodes with one switch case per node (or per run of small nodes), the nodes' inputs passed
+// through an empty asm so the compiler neither hoists their common subexpressions out of the cases nor
+// interleaves the cases: the register need is then one forest block's, not the sum of all of them held live
+// together (g3_lbl: 244 VGPRs, two waves per SIMD, instead of 256 + 212 AGPRs at one).  The same + - * /
+// run in the same order: the values are bit-identical to the straight-line form.
+#define SFX_NODE_SPLIT(R) (sizeof(R) == sizeof(double))
+__device__ __forceinline__ void sfx_opaque(double &v){ asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void sfx_opaque_s(int &v){ asm volatile("" : "+s"(v)); }
 
 #ifndef SFX_BLOCK
 #define SFX_BLOCK 256

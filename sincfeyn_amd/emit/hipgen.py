@@ -19,6 +19,8 @@ Transforms (text level; the kernels are regular machine-written C):
     double-double kernels, whose inlined bodies would take the compiler many minutes)
   * every call `FN(name)(args)`         -> `name<REAL>(args, SQ_AB)`  (chunk calls get no SQ_AB)
   * the Gauss-Legendre tables at file top -> `__constant__` arrays
+  * a point function whose forest nodes are all leaves gets a second body for the FP64 kernels, a loop
+    with one switch case per node (node_split; the register need of one forest block instead of all)
   * SQ_AB (the reference's thread-local "compute the |.| sums" flag, common.h:24-26) becomes a parameter;
     the runtime passes a compile-time constant, so with 0 the companion sums are dead code.
 The graph descriptor line `const graph_t G_<name>={...}` (common.h:38) gives name, Volkov string, loops L,
@@ -143,6 +145,80 @@ def transform(body):
     return code, names
 
 
+_PT_HEAD = 'template<class REAL> SFX_FN REAL pt_%s(const REAL *x, const REAL Vx, REAL *ab, const int SQ_AB){'
+_XLOAD_RE = re.compile(r'^  const REAL (x\d+)=x\[\d+\];$')
+_FINAL_RE = re.compile(r'^  \{ const REAL iV=1/\((n_\w+)_V\+Vx\);')
+
+
+def node_split(code, name):
+    """The point function once more as a loop over its forest nodes, for the FP64 kernels (see
+    SFX_NODE_SPLIT in sfx_integrand.hpp): one switch case per node, or per run of consecutive small nodes
+    (forest block under half the largest, so they keep their common subexpressions), each case followed
+    by that node's term of the final sum.  Applies when every node is a leaf and the final sum takes one
+    term per node in node order (the terms then run in the same order as in the straight-line form);
+    returns (code, number of cases), or (code unchanged, 0) when the point function has another shape."""
+    lines = code.split('\n')
+    head = _PT_HEAD % name
+    if head not in lines:
+        return code, 0
+    s = lines.index(head)
+    e = lines.index('}', s)
+    body = lines[s + 1:e]
+    nodes = [i for i, l in enumerate(body) if l.startswith('  /* node')]
+    if len(nodes) < 2 or any('kids ()' not in body[i] for i in nodes):
+        return code, 0
+    pre = body[:nodes[0]]
+    xs = [_XLOAD_RE.match(l) for l in pre]
+    if not pre or not all(xs):
+        return code, 0
+    xs = [m.group(1) for m in xs]
+    try:
+        t = body.index('  REAL tot=0, tota=0;')
+    except ValueError:
+        return code, 0
+    secs = [body[a:b] for a, b in zip(nodes, nodes[1:] + [t])]
+    fin, i, rest = [], 0, body[t + 1:]
+    while i < len(rest) and _FINAL_RE.match(rest[i]):
+        j = i
+        while not rest[j].endswith('tota+=ra; }'):
+            j += 1
+        fin.append((_FINAL_RE.match(rest[i]).group(1), rest[i:j + 1]))
+        i = j + 1
+    if rest[i:] != ['  *ab=tota; return tot;'] or len(fin) != len(secs):
+        return code, 0
+    cost = []
+    for sec, (nm, _) in zip(secs, fin):
+        if not sec[1].startswith('  REAL %s_W[' % nm):
+            return code, 0
+        calls = re.findall(r'\b(bk_\w+)<REAL>\(F_, T_\)', '\n'.join(sec))
+        c = len(sec)
+        for ch in calls:
+            hb = 'template<class REAL> SFX_FN void %s_b(const REAL *__restrict__ F, REAL *__restrict__ T){' % ch
+            if hb in lines:
+                b0 = lines.index(hb)
+                c += lines.index('}', b0) - b0
+        cost.append(c)
+    big = max(cost)
+    groups = []
+    for k, c in enumerate(cost):
+        if 2*c >= big or not groups or 2*cost[groups[-1][-1]] >= big:
+            groups.append([k])
+        else:
+            groups[-1].append(k)
+    xre = re.compile(r'\b(%s)\b' % '|'.join(xs))
+    sw = ['  } else {'] + pre + ['  REAL tot=0, tota=0;',
+          '  #pragma clang loop unroll(disable)',
+          '  for(int k_=0; k_<%d; k_++){ int kk_=k_; sfx_opaque_s(kk_); switch(kk_){' % len(groups)]
+    for gi, g in enumerate(groups):
+        sw += ['  case %d: {' % gi, '  ' + ' '.join('REAL q%s=%s; sfx_opaque(q%s);' % (x, x, x) for x in xs)]
+        for k in g:
+            sw += [xre.sub(lambda m: 'q' + m.group(1), l) for l in secs[k]] + fin[k][1]
+        sw += ['  } break;']
+    sw += ['  default: break;', '  } }', '  *ab=tota; return tot;', '  }']
+    out = lines[:s + 1] + ['  if constexpr(!SFX_NODE_SPLIT(REAL)){'] + body + sw + lines[e:]
+    return '\n'.join(out), len(groups)
+
+
 _OPS_RE = re.compile(r'(?<![eE])[-+*/](?![=+\-])')
 
 
@@ -169,6 +245,7 @@ def emit(src_path, out_dir):
     name = desc['name']
     if 'f_' + name not in names:
         raise EmitError('%s: no f_%s in the QREAL half' % (src_path, name))
+    code, ncases = node_split(code, name)
     inc = ['// generated by sincfeyn_amd/emit/hipgen.py from %s -- do not edit' % os.path.basename(src_path),
            '// the integrand of graph %s as function templates over REAL' % name]
     for xn, xl, xv, wn, wl, wv in tables:
@@ -206,7 +283,8 @@ def emit(src_path, out_dir):
         os.replace(out + '.tmp', out)
     out = os.path.join(out_dir, 'k_%s.hip' % name)
     meta = dict(desc, source=os.path.basename(src_path), functions=len(names),
-                chunk_functions=sum(1 for n in names if _is_chunk(n)), static_ops=count_ops(body))
+                chunk_functions=sum(1 for n in names if _is_chunk(n)), static_ops=count_ops(body),
+                node_split=ncases)
     with open(os.path.join(out_dir, 'k_%s.json' % name), 'w') as f:
         json.dump(meta, f, indent=1)
     return out, meta
