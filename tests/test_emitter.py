@@ -67,3 +67,84 @@ def test_op_counts_of_emitted_integrands():
         assert (d[g]['ops1'], d[g]['ops0'], d[g]['div1']) == (o1, o0, dv), g
     for g, c in d.items():
         assert c['ops0'] <= c['ops1'] and c['ops1'] > 100, g
+
+
+# a host build of an emitted integrand: REAL = double takes the node-split form (SFX_NODE_SPLIT), a
+# wrapper type W with the same IEEE double operations takes the straight-line form
+_HOST_DRIVER = r'''
+#include <cmath>
+#include <type_traits>
+#define __constant__
+#define SFX_FN static inline
+#define SFX_CHUNK static
+#define SFX_CHUNK_INLINE(R) true
+#define SFX_NODE_SPLIT(R) (std::is_same<R, double>::value)
+static inline void sfx_opaque(double &) {}
+static inline void sfx_opaque_s(int &) {}
+struct W { double v; W() = default; W(double x) : v(x) {} };
+static inline W operator+(W a, W b){ return W(a.v + b.v); }
+static inline W operator-(W a, W b){ return W(a.v - b.v); }
+static inline W operator*(W a, W b){ return W(a.v * b.v); }
+static inline W operator/(W a, W b){ return W(a.v / b.v); }
+static inline W operator-(W a){ return W(-a.v); }
+static inline W &operator+=(W &a, W b){ a.v += b.v; return a; }
+static inline W &operator-=(W &a, W b){ a.v -= b.v; return a; }
+static inline W &operator*=(W &a, W b){ a.v *= b.v; return a; }
+static inline W &operator/=(W &a, W b){ a.v /= b.v; return a; }
+static inline double sfx_abs(double x){ return std::fabs(x); }
+static inline W sfx_abs(W x){ return W(std::fabs(x.v)); }
+namespace k {
+#include KINC
+}
+extern "C" void run(int n, int D, const double *sf, int sqab, double *out){
+    for(int i = 0; i < n; i++){
+        double a1 = 0; double v1 = k::KF<double>(sf + (long)i*D, &a1, sqab);
+        W w[16]; for(int d = 0; d < D; d++) w[d] = W(sf[(long)i*D + d]);
+        W a2(0.0); W v2 = k::KF<W>(w, &a2, sqab);
+        out[4*i] = v1; out[4*i + 1] = a1; out[4*i + 2] = v2.v; out[4*i + 3] = a2.v;
+    }
+}
+'''
+
+
+def test_node_split_bit_identical(tmp_path):
+    """g3_lbl's point function, emitted a second time as one switch case per forest node for the FP64
+    kernels, computes bit for bit what its straight-line form computes (host build, g++ without FP
+    contraction, 300 random points, SQ_AB = 0 and 1), and both equal the reference kernel compiled
+    without contraction (oracle/_ref/libsqk_nc.so) -- the device point tests check the same on the GPU"""
+    import ctypes as C
+    import shutil
+    import subprocess
+    import numpy as np
+    from sincfeyn_amd import build
+    inc = os.path.join(ROOT, 'sincfeyn_amd', '_gen', 'k_g3_lbl.inc')
+    src = os.path.join(build.KERNELS, 'g3', 'g3_lbl.c')
+    if os.path.exists(src):
+        _, meta = hipgen.emit(src, str(tmp_path))
+        assert meta['node_split'] == 7              # six forest nodes of their own, the six small ones together
+        inc = str(tmp_path / 'k_g3_lbl.inc')
+    if not os.path.exists(inc) or not shutil.which('g++'):
+        pytest.skip('no emitted g3_lbl integrand or no g++')
+    text = open(inc).read()
+    assert 'if constexpr(!SFX_NODE_SPLIT(REAL)){' in text and text.count('  case ') == 7
+    drv = tmp_path / 'drv.cpp'
+    drv.write_text(_HOST_DRIVER)
+    so = tmp_path / 'drv.so'
+    subprocess.run(['g++', '-O1', '-ffp-contract=off', '-std=c++17', '-fPIC', '-shared', '-DKINC="%s"' % inc,
+                    '-DKF=f_g3_lbl', str(drv), '-o', str(so)], check=True)
+    L = C.CDLL(str(so))
+    rng = np.random.default_rng(7)
+    n, D = 300, 7
+    sf = np.exp(rng.uniform(-3, 3, size=(n, D)))
+    outs = {}
+    for sqab in (0, 1):
+        out = np.zeros((n, 4))
+        L.run(n, D, sf.ctypes.data_as(C.c_void_p), sqab, out.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(out[:, 0].view(np.uint64), out[:, 2].view(np.uint64)), sqab
+        assert np.array_equal(out[:, 1].view(np.uint64), out[:, 3].view(np.uint64)), sqab
+        outs[sqab] = out
+    import oracle as O
+    if os.path.exists(O.LIB_KERNELS_NC):
+        f, ab = O.NcKernels().fp64('g3_lbl', sf)
+        assert np.array_equal(f.view(np.uint64), outs[1][:, 0].view(np.uint64))
+        assert np.array_equal(ab.view(np.uint64), outs[1][:, 1].view(np.uint64))
