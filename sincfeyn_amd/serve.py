@@ -15,6 +15,7 @@ top from the reference's rule (main.c:211-215).
 import argparse
 import json
 import math
+import sys
 import threading
 import time
 
@@ -65,8 +66,10 @@ def percentile(xs, p):
     return xs[f] + (xs[c] - xs[f])*(k - f)
 
 
-def serve(stream, results=None):
-    """submit each job at its arrival time; returns the report dict (and fills results[i] = result dict)"""
+def serve(stream, results=None, progress=False):
+    """submit each job at its arrival time; returns the report dict (and fills results[i] = result dict);
+    with progress, one line per finished job on stderr (in submission order: job, graph, h, eps, queue and
+    run seconds), so a long stream shows where it is"""
     t0 = time.perf_counter()
     ids = [None]*len(stream)
     done = threading.Event()
@@ -82,10 +85,16 @@ def serve(stream, results=None):
     th = threading.Thread(target=client)
     th.start()
     lat, qs, ev = [], [], 0
+    names = [g['name'] for g in sfx.graphs()] if progress else None
     for i in range(len(stream)):
         while ids[i] is None:
             time.sleep(0.0005)
         r, tq, tr = sfx.wait(ids[i])
+        if progress:
+            a_, g, h, N, eps, top = stream[i]
+            print('job %d/%d %s h=%g eps=%g: queued %.3f s, ran %.3f s, %d points' % (
+                i + 1, len(stream), names[g], h, eps, tq, tr, r['nev']),
+                file=sys.stderr, flush=True)
         lat.append(tq + tr)
         qs.append(tq)
         ev += r['nev']
@@ -106,11 +115,12 @@ def main():
     ap.add_argument('--seed', type=int, default=1)
     ap.add_argument('--graphs', default='all2,all3')
     ap.add_argument('--workers', type=int, default=16)
+    ap.add_argument('--progress', action='store_true', help='one line per finished job on stderr')
     a = ap.parse_args()
     sfx.init(0)
     sfx.engine_start(a.workers, 0)
     graphs = sfx.select(*a.graphs.split(','))
-    rep = serve(make_stream(a.seed, a.rate, a.jobs, graphs))
+    rep = serve(make_stream(a.seed, a.rate, a.jobs, graphs), progress=a.progress)
     rep.update(rate=a.rate, seed=a.seed, workers=a.workers)
     print(json.dumps(rep))
 
