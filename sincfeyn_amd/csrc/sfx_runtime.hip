@@ -152,23 +152,39 @@ __device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, unsigne
     }
 }
 
+// The loads of LOAD_BATCH items are issued together before their sums (one memory latency per batch
+// instead of one per item); the items are added in the same order as one at a time.
+constexpr int LOAD_BATCH = 8;
+static_assert(TILE_ITEMS % LOAD_BATCH == 0, "tile items in whole load batches");
 __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
-    long base = (long)blockIdx.x*TILE;
+    const long base = (long)blockIdx.x*TILE + threadIdx.x;
     LevelPart t = {};
-    for(int j = 0; j < TILE_ITEMS; j++){
-        long i = base + j*TILE_THREADS + threadIdx.x;
-        if(i >= a.n) break;
-        uint8_t f = a.fl[i];
-        if(!(f & SFX_FL_VALID)) continue;
-        double v = a.v[i], x = v*a.pw, av = fabs(v);
-        unsigned cfl = a.cells[i >> a.D].fl;
-        const bool sk = a.abskip && (cfl & 1u);
-        t.nv++; t.nskip += sk ? 1u : 0u;
-        if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; t.nmiss += sk ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
-        dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
-        if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
-        else if(av >= a.thr) t.nkeep++;
-        else dacc_add(t.drop[0], t.drop[1], av*a.pw);
+    for(int j0 = 0; j0 < TILE_ITEMS; j0 += LOAD_BATCH){
+        if(base + (long)j0*TILE_THREADS >= a.n) break;
+        uint8_t fb[LOAD_BATCH], cb[LOAD_BATCH]; double vb[LOAD_BATCH];
+        #pragma unroll
+        for(int k = 0; k < LOAD_BATCH; k++){
+            const long i = base + (long)(j0 + k)*TILE_THREADS;
+            const bool in = i < a.n;
+            fb[k] = in ? a.fl[i] : 0;
+            vb[k] = in ? a.v[i] : 0.0;
+            cb[k] = in ? a.cells[i >> a.D].fl : 0;
+        }
+        #pragma unroll
+        for(int k = 0; k < LOAD_BATCH; k++){
+            const uint8_t f = fb[k];
+            if(!(f & SFX_FL_VALID)) continue;
+            const long i = base + (long)(j0 + k)*TILE_THREADS;
+            const double v = vb[k], x = v*a.pw, av = fabs(v);
+            const unsigned cfl = cb[k];
+            const bool sk = a.abskip && (cfl & 1u);
+            t.nv++; t.nskip += sk ? 1u : 0u;
+            if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; t.nmiss += sk ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
+            dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
+            if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
+            else if(av >= a.thr) t.nkeep++;
+            else dacc_add(t.drop[0], t.drop[1], av*a.pw);
+        }
     }
     tile_store(t, a.part, a.keepoff);
 }
