@@ -150,13 +150,16 @@ _XLOAD_RE = re.compile(r'^  const REAL (x\d+)=x\[\d+\];$')
 _FINAL_RE = re.compile(r'^  \{ const REAL iV=1/\((n_\w+)_V\+Vx\);')
 
 
-def node_split(code, name):
+def node_split(code, name, groups=None):
     """The point function once more as a loop over its forest nodes, for the FP64 kernels (see
     SFX_NODE_SPLIT in sfx_integrand.hpp): one switch case per node, or per run of consecutive small nodes
     (forest block under half the largest, so they keep their common subexpressions), each case followed
     by that node's term of the final sum.  Applies when every node is a leaf and the final sum takes one
     term per node in node order (the terms then run in the same order as in the straight-line form);
-    returns (code, number of cases), or (code unchanged, 0) when the point function has another shape."""
+    returns (code, number of cases), or (code unchanged, 0) when the point function has another shape.
+    groups (lists of consecutive node indices, in order) replaces the grouping rule; for g3_lbl, the rule's
+    six large nodes alone and the six small ones together measured best (244 VGPRs; two large nodes in
+    one case need more than 256 and drop to one wave per SIMD)."""
     lines = code.split('\n')
     head = _PT_HEAD % name
     if head not in lines:
@@ -199,12 +202,19 @@ def node_split(code, name):
                 c += lines.index('}', b0) - b0
         cost.append(c)
     big = max(cost)
+    if groups is not None:
+        return _node_cases(lines, s, e, body, pre, xs, secs, fin, groups)
     groups = []
     for k, c in enumerate(cost):
         if 2*c >= big or not groups or 2*cost[groups[-1][-1]] >= big:
             groups.append([k])
         else:
             groups[-1].append(k)
+    return _node_cases(lines, s, e, body, pre, xs, secs, fin, groups)
+
+
+def _node_cases(lines, s, e, body, pre, xs, secs, fin, groups):
+    """the point function's text with its second, switch-case body (node_split)"""
     xre = re.compile(r'\b(%s)\b' % '|'.join(xs))
     sw = ['  } else {'] + pre + ['  REAL tot=0, tota=0;',
           '  #pragma clang loop unroll(disable)',
