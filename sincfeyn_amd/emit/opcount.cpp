@@ -12,6 +12,11 @@
 #define SFX_FN inline
 #define SFX_CHUNK inline
 #define SFX_CHUNK_INLINE(R) true
+// the straight-line point function (the source's operations); the node-split body of the FP64 kernels
+// performs the same operations per node without the subexpressions shared between nodes
+#define SFX_NODE_SPLIT(R) false
+static inline void sfx_opaque(double &) {}
+static inline void sfx_opaque_s(int &) {}
 static long long N_ADD, N_MUL, N_DIV;
 struct oc {
     double v;

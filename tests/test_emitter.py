@@ -148,3 +148,18 @@ def test_node_split_bit_identical(tmp_path):
         f, ab = O.NcKernels().fp64('g3_lbl', sf)
         assert np.array_equal(f.view(np.uint64), outs[1][:, 0].view(np.uint64))
         assert np.array_equal(ab.view(np.uint64), outs[1][:, 1].view(np.uint64))
+
+
+def test_opcount_builds_on_node_split_integrand(tmp_path):
+    """the host op-count tool compiles every emitted integrand, the node-split g3_lbl included (it counts
+    the straight-line form: the source's 15,082 / 25,307 operations at SQ_AB = 0 / 1)"""
+    import shutil
+    import subprocess
+    gen = os.path.join(ROOT, 'sincfeyn_amd', '_gen')
+    if not os.path.exists(os.path.join(gen, 'k_g3_lbl.inc')) or not shutil.which('g++'):
+        pytest.skip('no emitted g3_lbl integrand or no g++')
+    exe = str(tmp_path / 'opc')
+    subprocess.run(['g++', '-O1', '-I' + gen, '-DSFX_INC="k_g3_lbl.inc"', '-DSFX_F=f_g3_lbl', '-DSFX_D=7',
+                    os.path.join(ROOT, 'sincfeyn_amd', 'emit', 'opcount.cpp'), '-o', exe], check=True)
+    d = json.loads(subprocess.run([exe], capture_output=True, text=True, check=True).stdout)
+    assert (d['ops0'], d['ops1'], d['div0']) == (15082, 25307, 25)
