@@ -40,16 +40,19 @@ def test_engine_jobs_match_oracle(sfx, oracle):
 
 
 @pytest.mark.gpu
-def test_serving_stream(sfx, oracle):
-    """a seeded Poisson stream (rate 40/s) of cheap jobs: every job served, each result identical to the
-    same job run alone, and each two-loop job's counts and value those of the CPU reference (the oracle,
-    no contraction)"""
+def test_serving_stream(sfx, oracle, capsys):
+    """a seeded Poisson stream (rate 40/s) of cheap jobs: every job served (one progress line each), each
+    result identical to the same job run alone, and each two-loop job's counts and value those of the CPU
+    reference (the oracle, no contraction)"""
     from sincfeyn_amd import serve
     sfx.engine_start(8, 0)
     graphs = sfx.select('all2', 'vp3')
     stream = serve.make_stream(20261203, 40.0, 24, graphs, hs=(1.0, 0.9), epss=(1e-10, 1e-12))
     res = [None]*len(stream)
-    rep = serve.serve(stream, res)
+    rep = serve.serve(stream, res, progress=True)
+    err = capsys.readouterr().err
+    assert [l.split()[1] for l in err.splitlines() if l.startswith('job ')] == \
+        ['%d/%d' % (i + 1, len(stream)) for i in range(len(stream))]
     print(rep, flush=True)
     assert rep['jobs'] == len(stream) and rep['latency_p99_s'] >= rep['latency_p50_s'] > 0
     names = {i: g['name'] for i, g in enumerate(sfx.graphs())}
