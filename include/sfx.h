@@ -41,6 +41,9 @@ extern "C" {
 #define SFX_ENOTINIT   -4   /* sfx_init not called */
 #define SFX_EUNSUP     -5   /* a mode this build does not support */
 #define SFX_ESTOPPED   -6   /* stopped after a checkpoint (sfx_opts_t.ckpt_stop_after); resume from the file */
+#define SFX_ETIMEOUT   -7   /* a wait passed its deadline (job wait, HBM budget); the message names the job,
+                               its stage and level */
+#define SFX_ECANCELED  -8   /* the job never ran: the engine was stopped while it was queued */
 
 typedef struct {
     char name[32];        /* "g3_00" */
@@ -188,9 +191,11 @@ int sfx_hsum(int graph, double h, int N, double eps, int top, const sfx_opts_t *
  * with defaults by the first submit), each with its own HIP stream and device buffers, runs the queued
  * jobs in arrival order, so jobs that arrive while others run are taken as workers free up and their
  * device work overlaps.  Each refinement level reserves the HBM bound of its next cell list from the
- * engine's budget (default 80% of the device memory) and waits while the budget is held by others.  A
- * job's result does not depend on what runs beside it.  No counterpart in the reference, which runs
- * graphs one after another in one process (main.c:243). */
+ * engine's budget (default 80% of the device memory) and waits while the budget is held by others; when
+ * every job that holds budget is itself waiting, the oldest waiter is admitted over the cap (no job waits
+ * on another forever).  A job's result does not depend on what runs beside it.  sfx_engine_stop fails the
+ * queued jobs with SFX_ECANCELED and waits for the running ones.  No counterpart in the reference, which
+ * runs graphs one after another in one process (main.c:243). */
 #define SFX_JOB_QUEUED  0
 #define SFX_JOB_RUNNING 1
 #define SFX_JOB_DONE    2
@@ -199,8 +204,14 @@ int sfx_engine_start(int workers, double hbm_budget_bytes);      /* <= 0: 8 work
 int sfx_engine_stop(void);
 int sfx_job_submit(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, long *job);
 int sfx_job_poll(long job, int *state);
-/* blocks until the job has finished; *t_queue = seconds from submit to start, *t_run = start to end */
+/* blocks until the job has finished, at most the engine's wait deadline (default 3600 s; then
+ * SFX_ETIMEOUT, the job keeps running); *t_queue = seconds from submit to start, *t_run = start to end */
 int sfx_job_wait(long job, sfx_result_t *res, double *t_queue, double *t_run);
+/* the same with a deadline of its own (seconds, <= 0: none) */
+int sfx_job_wait_for(long job, double timeout_s, sfx_result_t *res, double *t_queue, double *t_run);
+/* the deadlines of sfx_job_wait and of a job's wait for HBM budget (seconds; <= 0: none; defaults 3600
+ * and 600): a wait that passes one fails with SFX_ETIMEOUT naming the job, its stage and level */
+int sfx_engine_deadlines(double job_wait_s, double budget_wait_s);
 int sfx_job_release(long job);
 
 /* ---- measurement ----

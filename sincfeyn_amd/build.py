@@ -108,12 +108,16 @@ def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None):
         names = sorted(os.path.basename(p)[2:-4] for p in glob.glob(os.path.join(GEN, 'k_*.hip')))
         if only:
             names = [n for n in names if n in only]
-        log('kernel sources absent; compiling the %d emitted integrands in %s' % (len(names), GEN))
-    if not names:
-        raise RuntimeError('no integrand kernels to build')
+        missing = [n for n in names if not all(os.path.exists(os.path.join(GEN, 'k_%s.%s' % (n, x))) for x in ('inc', 'json'))]
+        if not names or missing:
+            raise RuntimeError('cannot build: the sincFeyn kernel sources are not at %s (set SFX_KERNELS to its kernels/ '
+                               'directory) and %s holds %s' % (KERNELS, GEN, 'no emitted integrands' if not names else
+                               'incomplete emitted integrands (no .inc/.json for %s)' % ', '.join(missing)))
+        log('kernel sources absent; compiling the %d emitted integrands already in %s' % (len(names), GEN))
     reg = write_registry(names)
     hdrs = [os.path.join(CSRC, h) for h in ('sfx_integrand.hpp', 'sfx_dd.hpp', 'sfx_qd.hpp', 'sfx_abi_types.h')]
-    rt_hdrs = hdrs + [os.path.join(CSRC, 'sfx_device.hpp'), os.path.join(CSRC, 'sfx_grow.inc'), os.path.join(CSRC, 'sfx_scan.inc'), os.path.join(CSRC, 'sfx_ckpt.inc'), os.path.join(ROOT, 'include', 'sfx.h')]
+    rt_hdrs = hdrs + [os.path.join(CSRC, h) for h in ('sfx_device.hpp', 'sfx_engine.hpp', 'sfx_grow.inc', 'sfx_scan.inc', 'sfx_ckpt.inc')]
+    rt_hdrs.append(os.path.join(ROOT, 'include', 'sfx.h'))
     units = [(os.path.join(GEN, 'k_%s.hip' % n), os.path.join(OBJ, 'k_%s.o' % n), hdrs + [os.path.join(GEN, 'k_%s.inc' % n)])
              for n in names]
     units.append((reg, os.path.join(OBJ, 'sfx_registry.o'), hdrs))

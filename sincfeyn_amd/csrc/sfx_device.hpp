@@ -27,9 +27,12 @@ struct Lib {
 Lib &lib();
 int require_init();
 
-// a device buffer that grows on demand (never shrinks while the library is up)
+// a device buffer that grows on demand.  Bound to a stream (the context's), its memory is allocated and
+// freed in that stream's order (hipMallocAsync / hipFreeAsync: no device-wide synchronisation, so a
+// worker that grows a buffer does not wait for every other worker's kernels); unbound, with hipMalloc /
+// hipFree.  reserve() does not keep the contents.
 struct DevBuf {
-    void *p = nullptr; size_t cap = 0;
+    void *p = nullptr; size_t cap = 0; hipStream_t s = nullptr; bool bound = false;
     int reserve(size_t bytes);
     void release();
     template<class T> T *as() const { return (T*)p; }

@@ -22,6 +22,7 @@
 #include <map>
 #include <atomic>
 #include "sfx_device.hpp"
+#include "sfx_engine.hpp"
 
 namespace sfx {
 
@@ -41,6 +42,8 @@ struct JobBufs {
         for(DevBuf *b : all) f(*b);
     }
     void release(){ each([](DevBuf &b){ b.release(); }); }
+    // allocate and free every buffer in the order of stream s (released buffers only)
+    void bind(hipStream_t s){ each([s](DevBuf &b){ if(!b.p){ b.s = s; b.bound = true; } }); }
     // free the buffers above `keep` bytes (a large job's cell lists and coarsest-level tables), so what a
     // worker holds between jobs stays small and the engine's HBM budget describes the device
     void trim(size_t keep){ each([keep](DevBuf &b){ if(b.cap > keep) b.release(); }); }
@@ -95,7 +98,7 @@ static int upload(Ctx &X, hipStream_t s, void *dst, const void *src, size_t byte
     SFX_HIP(hipStreamSynchronize(s));
     return SFX_OK;
 }
-static Ctx &default_ctx(){ static Ctx C; C.s = lib().stream; return C; }
+static Ctx &default_ctx(){ static Ctx C; if(C.s != lib().stream){ C.s = lib().stream; C.B.bind(C.s); } return C; }
 
 // ---- profiling: HIP events on the library stream around each kernel, accumulated per (graph, kind)
 struct Prof {
@@ -132,6 +135,7 @@ struct Job {
     const sfx_graph_entry *g; int gi, D, K, top; double h, eps;
     sfx_opts_t o; sfx_eval_params_t P;
     hipStream_t s; Ctx *ctx;
+    JobProgress *pg = nullptr;         // where the sum is (stage, level, block), for the engine's messages
 };
 
 // ---------------------------------------------------------------------------------------------------
@@ -230,7 +234,8 @@ static void set_params(Job &J, double abserr){
 static int reserve_preserving(DevBuf &b, size_t used, size_t want, hipStream_t s){
     if(want <= b.cap) return SFX_OK;
     size_t c = b.cap ? b.cap : 4096; while(c < want) c *= 2;
-    DevBuf t; int rc = t.reserve(c); if(rc) return rc;
+    DevBuf t; t.s = b.s; t.bound = b.bound;
+    int rc = t.reserve(c); if(rc) return rc;
     if(used) SFX_HIP(hipMemcpyAsync(t.p, b.p, used, hipMemcpyDeviceToDevice, s));
     SFX_HIP(hipStreamSynchronize(s));
     b.release(); b = t;
@@ -310,6 +315,7 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     for(long c0 = ls.c0; c0 < ncur; c0 += Bc){
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
+        J.pg->block.store(c0/Bc);
         const bool pon = prof().on;
         if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
         // the block's cells in two lists by class, each run by the integrand compiled for its class; the
@@ -378,7 +384,6 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
 
 using namespace sfx;
 
-namespace sfx { void engine_shutdown(); }
 extern "C" void sfx_release_job_buffers(void){ engine_shutdown(); default_ctx().B.release(); default_ctx().H.release(); }
 
 extern "C" int sfx_prof_enable(int on){
@@ -441,12 +446,14 @@ static int check_job(int graph, double h, int N, double eps, int top){
     return SFX_OK;
 }
 
-namespace sfx { void budget_acquire(double bytes, double mine); void budget_release(double bytes); }
-
-// hsum of one graph on the context's stream (arguments checked by the caller)
-static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res){
+// hsum of one graph on the context's stream (arguments checked by the caller); pg (may be null) receives
+// the stage and level the sum is at
+static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, sfx_result_t *res,
+                     JobProgress *pg){
     int rc;
     Job J; J.gi = graph; J.g = SFX_GRAPHS[graph]; J.D = J.g->D; J.h = h; J.eps = eps; J.top = top; J.ctx = &X; J.s = X.s;
+    JobProgress pg0; J.pg = pg ? pg : &pg0;
+    char who[96]; snprintf(who, sizeof who, "%s h=%g N=%d eps=%g", J.g->name, h, N, eps);
     if(opts) J.o = *opts; else sfx_opts_default(&J.o);
     if(J.o.qd_fallback && !J.g->has_qd){ set_error("%s: no quad-double kernel (two-loop graphs only)", J.g->name); return SFX_EUNSUP; }
     int m = 1 << (top + 1); J.K = (N - 1)/2; J.K = ((J.K + m - 1)/m)*m;      // lattice.c:16
@@ -521,17 +528,19 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
     set_params(J, abserr);
     res->esok = resumed ? C.esok : J.D < 12 && !J.o.box_scan && top > 0;   // sum.c:597, 614
     CkRun ckr; ckr.ck = &ck; ckr.res = res; ckr.Slev = Slev; ckr.t0 = t0;
-    double held = (double)ncur*sizeof(sfx_cell_t);           // HBM reservation of this job's cell lists
-    budget_acquire(held, 0);
+    BudgetHold bh;                                            // HBM reservation of this job's cell lists
+    if((rc = budget_acquire(bh, (double)ncur*sizeof(sfx_cell_t), who, J.pg))) return rc;
     for(int lev = lev0; lev >= 0; lev--){
         long nnext = 0; LevelAcc A;
         const LevelStart none, &ls = lev == lev0 ? ls0 : none;
+        J.pg->set(STAGE_LEVEL, lev);
         // the bound of the next list: every child of the cells still to do kept, plus those kept already
         double want = lev > 0 ? ((double)(ncur - ls.c0)*(double)(1L << J.D) + (ls.nxt ? (double)ls.nxt->size() : 0))*sizeof(sfx_cell_t) : 0;
-        budget_acquire(want, held);
+        if((rc = budget_acquire(bh, want, who, J.pg))){ budget_release(bh, bh.held); return rc; }
         ckr.nl = nl;
-        if((rc = refine_level(J, lev, ncur, &nnext, ref, &A, ls, ckr))){ budget_release(held + want); return rc; }
-        if(lev > 0){ double now_ = (double)nnext*sizeof(sfx_cell_t); budget_release(held + want - now_); held = now_; }
+        if((rc = refine_level(J, lev, ncur, &nnext, ref, &A, ls, ckr))){ budget_release(bh, bh.held); return rc; }
+        // keep the reservation of the list the next level reads
+        if(lev > 0) budget_release(bh, bh.held - (double)nnext*sizeof(sfx_cell_t));
         double S = A.s[0] + A.s[1];
         res->prune += A.drop[0] + A.drop[1];
         res->nev += (long)A.nv; res->nq += (long)A.nq; res->unresdrop += (long)A.nd;
@@ -548,21 +557,21 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
                 // the level-end checkpoint (sum.c:798): the next level's cells, nothing of it done yet
                 CkState c; c.lev = lev - 1; c.next = 0;
                 ck_fill_run(c, res, Slev, nl, now() - t0);
-                if((rc = download_cells(B.cur, ncur, J.s, ckr.cur))) { budget_release(held); return rc; }
+                if((rc = download_cells(B.cur, ncur, J.s, ckr.cur))) { budget_release(bh, bh.held); return rc; }
                 ckr.cur_lev = lev - 1;
                 cells_to_pts(ckr.cur, J.D, c.cur);
                 c.cnt[1] = res->abskipped; c.cnt[2] = res->abmiss;
                 ck_write(ck, c);
                 res->ckpts_written = ck.written;
                 if(ck.stop_after && ck.written >= ck.stop_after){
-                    budget_release(held);
+                    budget_release(bh, bh.held);
                     set_error("%s: stopped after checkpoint %d (%s)", J.g->name, ck.written, ck.path.c_str());
                     return SFX_ESTOPPED;
                 }
             }
         }
     }
-    budget_release(held);
+    budget_release(bh, bh.held);
     ck_remove(ck);
     res->levels = nl;
     res->S[0] = Slev[nl - 1]; res->S[1] = nl > 1 ? Slev[nl - 2] : NAN; res->S[2] = nl > 2 ? Slev[nl - 3] : NAN;
@@ -576,173 +585,79 @@ extern "C" int sfx_hsum(int graph, double h, int N, double eps, int top, const s
     if((rc = require_init())) return rc;
     if(!res){ set_error("sfx_hsum: null result"); return SFX_EINVAL; }
     if((rc = check_job(graph, h, N, eps, top))) return rc;
-    return hsum_core(default_ctx(), graph, h, N, eps, top, opts, res);
+    return hsum_core(default_ctx(), graph, h, N, eps, top, opts, res, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------------
-// job engine: a pool of workers, each with its own HIP stream and device buffers, taking (graph, h, N,
-// eps, top) jobs from a FIFO queue.  A job's coarsest level runs on its worker's host thread (device
-// evaluations on the worker's stream) while other workers' refinements run on the device, so host and
-// device work of different jobs overlap and kernels of different jobs run concurrently.  Admission of a
-// refinement level is by its HBM footprint: before a non-last level, the job reserves the bound of its
-// next cell list (every child kept: n_cells 2^D x 16 B) from the engine's budget and waits while other
-// jobs hold it; the reservation shrinks to the actual list after the level.
+// job engine (sfx_engine.hpp): the library's backend runs hsum_core on the worker's own context
 namespace sfx {
 
-struct Budget {
-    std::mutex mu; std::condition_variable cv; double cap = 0, used = 0;
-};
-static Budget &budget(){ static Budget B; return B; }
-// `mine` is what the caller already holds: a job waits only for the reservations of other jobs, and is
-// admitted when it would be the only holder even if it alone exceeds the budget
-void budget_acquire(double bytes, double mine){
-    Budget &B = budget(); if(bytes <= 0) return;
-    std::unique_lock<std::mutex> lk(B.mu);
-    B.cv.wait(lk, [&]{ return B.cap <= 0 || B.used - mine <= 0 || B.used + bytes <= B.cap; });
-    B.used += bytes;
+static void *ectx_create(){
+    Ctx *c = new Ctx();
+    hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+    if(e != hipSuccess){ delete c; hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); return nullptr; }
+    c->B.bind(c->s);
+    return c;
 }
-void budget_release(double bytes){
-    Budget &B = budget(); if(bytes <= 0) return;
-    { std::lock_guard<std::mutex> lk(B.mu); B.used -= bytes; if(B.used < 0) B.used = 0; }
-    B.cv.notify_all();
+static void ectx_destroy(void *p){
+    Ctx *c = (Ctx*)p;
+    (void)hipStreamSynchronize(c->s); c->B.release(); (void)hipStreamSynchronize(c->s); c->H.release();
+    for(hipEvent_t &v : c->ev) if(v) (void)hipEventDestroy(v);
+    (void)hipStreamDestroy(c->s); delete c;
 }
-
-struct EJob {
-    int graph = 0, N = 0, top = 0; double h = 0, eps = 0; sfx_opts_t o;
-    int state = SFX_JOB_QUEUED, rc = 0; char err[256] = "";
-    sfx_result_t res;
-    double t_submit = 0, t_start = 0, t_end = 0;
-};
-struct Engine {
-    std::mutex mu; std::condition_variable cv, done;
-    std::deque<long> queue;
-    std::map<long, EJob> jobs;
-    std::vector<std::thread> workers;
-    std::vector<Ctx*> ctx;
-    bool stop = false; long next = 1; int running = 0;
-    // at process exit the idle workers are woken and joined (a joinable std::thread must not be destroyed);
-    // the HIP objects are left to the runtime's own teardown
-    ~Engine(){
-        { std::lock_guard<std::mutex> lk(mu); stop = true; }
-        cv.notify_all();
-        for(std::thread &t : workers) if(t.joinable()) t.join();
-    }
-};
-static Engine &eng(){ static Engine E; return E; }
-
-static void worker_main(int w){
-    Engine &E = eng();
-    (void)hipSetDevice(lib().device);
-    Ctx &X = *E.ctx[w];
-    for(;;){
-        long id; EJob *j;
-        {
-            std::unique_lock<std::mutex> lk(E.mu);
-            E.cv.wait(lk, [&]{ return E.stop || !E.queue.empty(); });
-            if(E.stop) return;
-            id = E.queue.front(); E.queue.pop_front();
-            j = &E.jobs[id]; j->state = SFX_JOB_RUNNING; j->t_start = now(); E.running++;
-        }
-        sfx_result_t r;
-        int rc = hsum_core(X, j->graph, j->h, j->N, j->eps, j->top, &j->o, &r);
-        (void)hipStreamSynchronize(X.s);
-        X.B.trim((size_t)256 << 20); X.H.trim((size_t)64 << 20);
-        {
-            std::lock_guard<std::mutex> lk(E.mu);
-            j->res = r; j->rc = rc; j->t_end = now();
-            if(rc) snprintf(j->err, sizeof j->err, "%s", sfx_last_error());
-            j->state = rc ? SFX_JOB_FAILED : SFX_JOB_DONE; E.running--;
-        }
-        E.done.notify_all();
-    }
+static int erun(void *p, EJob &j){
+    Ctx &X = *(Ctx*)p;
+    int rc = hsum_core(X, j.graph, j.h, j.N, j.eps, j.top, &j.o, &j.res, &j.prog);
+    if(rc) snprintf(j.err, sizeof j.err, "%s", sfx_last_error());
+    return rc;
 }
-
-static int engine_start(int workers, double budget_bytes){
-    Engine &E = eng();
-    std::lock_guard<std::mutex> lk(E.mu);
-    if(!E.workers.empty()) return SFX_OK;
-    if(workers <= 0) workers = 8;
-    if(workers > 64) workers = 64;
-    budget().cap = budget_bytes > 0 ? budget_bytes : 0.8*(double)lib().prop.totalGlobalMem;
-    E.stop = false;
-    for(int w = 0; w < workers; w++){
-        Ctx *c = new Ctx();
-        hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
-        if(e != hipSuccess){ delete c; return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); }
-        E.ctx.push_back(c);
-    }
-    for(int w = 0; w < workers; w++) E.workers.emplace_back(worker_main, w);
-    return SFX_OK;
+// free the buffers above 256 MB after each job (a large job's cell lists and coarsest-level tables), so
+// what a worker holds between jobs stays small and the engine's HBM budget describes the device; the
+// frees are ordered on the worker's stream and do not synchronise the device
+static void eafter(void *p){
+    Ctx &X = *(Ctx*)p;
+    X.B.trim((size_t)256 << 20);
+    (void)hipStreamSynchronize(X.s);
+    X.H.trim((size_t)64 << 20);
 }
-
-void engine_shutdown(){
-    Engine &E = eng();
-    { std::lock_guard<std::mutex> lk(E.mu); E.stop = true; }
-    E.cv.notify_all();
-    for(std::thread &t : E.workers) t.join();
-    E.workers.clear();
-    for(Ctx *c : E.ctx){ (void)hipStreamSynchronize(c->s); c->B.release(); c->H.release(); for(hipEvent_t &v : c->ev) if(v) (void)hipEventDestroy(v);
-                         (void)hipStreamDestroy(c->s); delete c; }
-    E.ctx.clear();
-    std::lock_guard<std::mutex> lk(E.mu);
-    E.queue.clear(); E.jobs.clear(); E.running = 0;
-    { std::lock_guard<std::mutex> lb(budget().mu); budget().cap = 0; }   // no budget outside the engine
-}
+static void engine_bind(){ EngineBackend &b = eng().be; if(!b.run){ b.ctx_create = ectx_create; b.ctx_destroy = ectx_destroy; b.run = erun; b.after_job = eafter; } }
 
 }  // namespace sfx
 
 extern "C" int sfx_engine_start(int workers, double hbm_budget_bytes){
     int rc; if((rc = require_init())) return rc;
-    return engine_start(workers, hbm_budget_bytes);
+    { std::lock_guard<std::mutex> lk(eng().mu); engine_bind(); }
+    return engine_start(workers, hbm_budget_bytes, (double)lib().prop.totalGlobalMem);
 }
 extern "C" int sfx_engine_stop(void){ engine_shutdown(); return SFX_OK; }
+extern "C" int sfx_engine_deadlines(double job_wait_s, double budget_wait_s){
+    { std::lock_guard<std::mutex> lk(eng().mu); eng().wait_deadline_s = job_wait_s; }
+    { std::lock_guard<std::mutex> lk(budget().mu); budget().deadline_s = budget_wait_s; }
+    budget().cv.notify_all();
+    return SFX_OK;
+}
 
 extern "C" int sfx_job_submit(int graph, double h, int N, double eps, int top, const sfx_opts_t *opts, long *job){
     int rc;
     if((rc = require_init())) return rc;
     if(!job){ set_error("sfx_job_submit: null job pointer"); return SFX_EINVAL; }
     if((rc = check_job(graph, h, N, eps, top))) return rc;
-    if(eng().workers.empty() && (rc = engine_start(0, 0))) return rc;
-    Engine &E = eng();
-    {
-        std::lock_guard<std::mutex> lk(E.mu);
-        long id = E.next++;
-        EJob &j = E.jobs[id];
-        j.graph = graph; j.h = h; j.N = N; j.eps = eps; j.top = top; j.t_submit = now();
-        if(opts) j.o = *opts; else sfx_opts_default(&j.o);
-        E.queue.push_back(id);
-        *job = id;
-    }
-    E.cv.notify_one();
-    return SFX_OK;
+    EJob p;
+    p.graph = graph; p.h = h; p.N = N; p.eps = eps; p.top = top;
+    if(opts) p.o = *opts; else sfx_opts_default(&p.o);
+    snprintf(p.name, sizeof p.name, "%s", SFX_GRAPHS[graph]->name);
+    { std::lock_guard<std::mutex> lk(eng().mu); engine_bind(); }
+    return engine_submit(p, job, (double)lib().prop.totalGlobalMem);
 }
 
-extern "C" int sfx_job_poll(long job, int *state){
-    Engine &E = eng(); std::lock_guard<std::mutex> lk(E.mu);
-    auto it = E.jobs.find(job);
-    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
-    if(state) *state = it->second.state;
-    return SFX_OK;
-}
+extern "C" int sfx_job_poll(long job, int *state){ return engine_poll(job, state); }
 
 extern "C" int sfx_job_wait(long job, sfx_result_t *res, double *t_queue, double *t_run){
-    Engine &E = eng(); std::unique_lock<std::mutex> lk(E.mu);
-    auto it = E.jobs.find(job);
-    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
-    EJob &j = it->second;
-    E.done.wait(lk, [&]{ return j.state == SFX_JOB_DONE || j.state == SFX_JOB_FAILED; });
-    if(res) *res = j.res;
-    if(t_queue) *t_queue = j.t_start - j.t_submit;
-    if(t_run) *t_run = j.t_end - j.t_start;
-    if(j.rc){ set_error("%s", j.err); return j.rc; }
-    return SFX_OK;
+    double d; { std::lock_guard<std::mutex> lk(eng().mu); d = eng().wait_deadline_s; }
+    return engine_wait(job, d, res, t_queue, t_run);
+}
+extern "C" int sfx_job_wait_for(long job, double timeout_s, sfx_result_t *res, double *t_queue, double *t_run){
+    return engine_wait(job, timeout_s, res, t_queue, t_run);
 }
 
-extern "C" int sfx_job_release(long job){
-    Engine &E = eng(); std::lock_guard<std::mutex> lk(E.mu);
-    auto it = E.jobs.find(job);
-    if(it == E.jobs.end()){ set_error("no job %ld", job); return SFX_EINVAL; }
-    if(it->second.state == SFX_JOB_QUEUED || it->second.state == SFX_JOB_RUNNING){ set_error("job %ld not finished", job); return SFX_EINVAL; }
-    E.jobs.erase(it);
-    return SFX_OK;
-}
+extern "C" int sfx_job_release(long job){ return engine_release(job); }

@@ -33,13 +33,17 @@ int DevBuf::reserve(size_t bytes){
     size_t want = cap ? cap : 4096;
     while(want < bytes && want < ((size_t)64 << 20)) want *= 2;
     if(want < bytes) want = (bytes + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
-    if(p){ hipError_t e = hipFree(p); p = nullptr; cap = 0; if(e != hipSuccess) return hip_fail(e, "hipFree", __FILE__, __LINE__); }
-    hipError_t e = hipMalloc(&p, want);
-    if(e != hipSuccess){ p = nullptr; set_error("hipMalloc(%zu bytes): %s", want, hipGetErrorString(e)); return SFX_ENOMEM; }
+    if(p){
+        hipError_t e = bound ? hipFreeAsync(p, s) : hipFree(p);
+        p = nullptr; cap = 0;
+        if(e != hipSuccess) return hip_fail(e, bound ? "hipFreeAsync" : "hipFree", __FILE__, __LINE__);
+    }
+    hipError_t e = bound ? hipMallocAsync(&p, want, s) : hipMalloc(&p, want);
+    if(e != hipSuccess){ p = nullptr; set_error("%s(%zu bytes): %s", bound ? "hipMallocAsync" : "hipMalloc", want, hipGetErrorString(e)); return SFX_ENOMEM; }
     cap = want;
     return SFX_OK;
 }
-void DevBuf::release(){ if(p) (void)hipFree(p); p = nullptr; cap = 0; }
+void DevBuf::release(){ if(p) (void)(bound ? hipFreeAsync(p, s) : hipFree(p)); p = nullptr; cap = 0; }
 
 // ---------------------------------------------------------------------------------------------------
 // double-double helpers for the reductions (the sum of two doubles exactly, then renormalised)
@@ -350,6 +354,13 @@ extern "C" int sfx_init(int device){
     SFX_HIP(hipSetDevice(device));
     SFX_HIP(hipGetDeviceProperties(&L.prop, device));
     SFX_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    // the stream-ordered allocations of the contexts come from the device's default pool; it keeps up to
+    // 16 GB of freed memory for reuse instead of returning it to the driver at every synchronisation
+    {
+        hipMemPool_t pool; uint64_t keep = (uint64_t)16 << 30;
+        SFX_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+        SFX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
     L.device = device; L.init = true;
     char b[512];
     snprintf(b, sizeof b, "%s (%s), %d CUs, %.0f MHz, %.1f GB HBM, %d graphs", L.prop.name, L.prop.gcnArchName,

@@ -66,21 +66,24 @@ def percentile(xs, p):
     return xs[f] + (xs[c] - xs[f])*(k - f)
 
 
-def serve(stream, results=None, progress=False):
+def serve(stream, results=None, progress=False, wait_timeout=600.0):
     """submit each job at its arrival time; returns the report dict (and fills results[i] = result dict);
     with progress, one line per finished job on stderr (in submission order: job, graph, h, eps, queue and
-    run seconds), so a long stream shows where it is"""
+    run seconds), so a long stream shows where it is.  A job not finished wait_timeout seconds after the
+    wait for it began raises (SfxError naming its stage and level) instead of blocking."""
     t0 = time.perf_counter()
     ids = [None]*len(stream)
-    done = threading.Event()
+    failed = []                                 # the client thread's exception, re-raised by the waiting loop
 
     def client():
-        for i, (a, g, h, N, eps, top) in enumerate(stream):
-            dt = t0 + a - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids[i] = sfx.submit(g, h, N, eps, top)
-        done.set()
+        try:
+            for i, (a, g, h, N, eps, top) in enumerate(stream):
+                dt = t0 + a - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = sfx.submit(g, h, N, eps, top)
+        except BaseException as e:             # noqa: BLE001 -- handed to the main thread
+            failed.append(e)
 
     th = threading.Thread(target=client)
     th.start()
@@ -88,8 +91,11 @@ def serve(stream, results=None, progress=False):
     names = [g['name'] for g in sfx.graphs()] if progress else None
     for i in range(len(stream)):
         while ids[i] is None:
+            if failed:
+                th.join()
+                raise failed[0]
             time.sleep(0.0005)
-        r, tq, tr = sfx.wait(ids[i])
+        r, tq, tr = sfx.wait(ids[i], timeout=wait_timeout)
         if progress:
             a_, g, h, N, eps, top = stream[i]
             print('job %d/%d %s h=%g eps=%g: queued %.3f s, ran %.3f s, %d points' % (

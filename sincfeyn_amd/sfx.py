@@ -275,7 +275,9 @@ def _engine_sigs():
     L.sfx_job_submit.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(Opts), C.POINTER(C.c_long)]
     L.sfx_job_poll.argtypes = [C.c_long, C.POINTER(C.c_int)]
     L.sfx_job_wait.argtypes = [C.c_long, C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.sfx_job_wait_for.argtypes = [C.c_long, C.c_double, C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.sfx_job_release.argtypes = [C.c_long]
+    L.sfx_engine_deadlines.argtypes = [C.c_double, C.c_double]
     L._engine_sigs = True
     return L
 
@@ -286,6 +288,11 @@ def engine_start(workers=0, hbm_budget_bytes=0.0):
 
 def engine_stop():
     _check(_engine_sigs().sfx_engine_stop())
+
+
+def engine_deadlines(job_wait_s=3600.0, budget_wait_s=600.0):
+    """the deadlines of a job wait and of a job's wait for HBM budget (seconds, <= 0: none)"""
+    _check(_engine_sigs().sfx_engine_deadlines(job_wait_s, budget_wait_s))
 
 
 def submit(graph, h, N, eps, top=None, **kw):
@@ -305,11 +312,18 @@ def poll(job):
     return s.value
 
 
-def wait(job, release=True):
-    """(result dict, seconds queued, seconds running) of a job, blocking until it is done"""
+def wait(job, release=True, timeout=None):
+    """(result dict, seconds queued, seconds running) of a job, blocking until it is done -- at most
+    `timeout` seconds (None: the engine's wait deadline), then SfxError (SFX_ETIMEOUT) naming the job's
+    stage and level; the job keeps running and can be waited for again"""
     L = _engine_sigs()
     r = Result(); tq = C.c_double(); tr = C.c_double()
-    rc = L.sfx_job_wait(job, C.byref(r), C.byref(tq), C.byref(tr))
+    if timeout is None:
+        rc = L.sfx_job_wait(job, C.byref(r), C.byref(tq), C.byref(tr))
+    else:
+        rc = L.sfx_job_wait_for(job, float(timeout), C.byref(r), C.byref(tq), C.byref(tr))
+    if rc == -7:                                # SFX_ETIMEOUT: the job is still there
+        _check(rc)
     if release:
         L.sfx_job_release(job)
     _check(rc)
