@@ -1,42 +1,57 @@
 #!/usr/bin/env python3
-"""bench.py -- Sinc-sum points/s of the MI355X evaluator on sincFeyn's three-loop benchmark.
+"""bench.py -- Sinc-sum points/s of the MI355X evaluator on sincFeyn's batched heterogeneous workload.
 
-Workload (one step): the 41 three-loop vertex integrals (`all3`: g3_00..g3_27, g3_lbl, v3_00..v3_07,
-v3_p4Sc/Sm/Vc/Vm) at h = 0.7, N = 139 (the repo default 2*ceil(48/h)+1; K rounds to 72), eps = 1e-12,
-top = 2, each graph's hierarchical sum run to completion on the device (one graph after another: the
-plain per-graph path).  A point is one lattice-point evaluation as the reference counts it (res_t.nev).
+Workload: the 46 two- and three-loop integrals (`all2` + `all3`: g2_00..g2_03, v2_00, g3_00..g3_27,
+g3_lbl, v3_00..v3_07, v3_p4Sc/Sm/Vc/Vm) at h = 1.0, 0.8 and 0.7 (N = 97, 121, 139: the repo default
+2*ceil(48/h)+1; top 1, 1, 2 by the reference's rule), eps = 1e-12 -- 138 hierarchical sums with lattice
+depths and dimensions from 3 to 7, 2.66e11 lattice-point evaluations.  A point is one evaluation as the
+reference counts it (res_t.nev).
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--no-cpu-baseline]
+A step is one tenth of the 138 jobs (the jobs sorted widest first and dealt round-robin into ten
+slices; step i runs slice i mod 10), so ten steps are one pass over the whole workload.  The batched
+path submits the steps' jobs to the library's job engine (a pool of workers with their own streams,
+continuous batching); `value` is its rate over the timed steps.
 
-Two paths over the same jobs: the plain per-graph path (one hierarchical sum after another) and the
-batched path (all jobs submitted to the library's job engine, which runs them on a pool of workers with
-their own streams); `value` is the batched rate, `per_graph` the plain one, and every batched result is
-checked to equal its per-graph result.
+Beside it, on the same machine:
+  per_graph    one pass of the same 138 jobs as plain per-graph sums (one after another on one stream)
+  plain_gpu    the reference's own host driver built as its GPU build (driver/sum.c, sumkey.h, eval.c with
+               -DSQ_GPU: grow, dilation, prep, sums and double-double fallback on the host, each batch
+               through the gpu.h back end -- here this library's, src/sfx_gpu.c), one graph per launch,
+               on a bounded sample of the jobs; the batched and per-graph paths on the same sample
+  cpu_baseline the reference's OpenMP CPU build on a bounded sample, on this machine's host cores
+  roofline     the dominant kernel of a profiled per-graph pass over the h = 0.7 jobs of the three-loop
+               integrals: FP64 operations per point x points per launch / mean launch duration (HIP
+               events on the library's stream), algorithmic and executed (committed VALU counters)
+  reference    each batched result against the reference's own results (committed fixtures written by
+               tests/golden/make_fixtures.py): counts exact, values within 1e-12
 
-Prints one JSON line: points/s (value), per-job latency percentiles, the roofline of the dominant
-kernel (FP64 operations per point of each integrand x points per launch / mean launch duration, from
-HIP events on the library's stream), and the reference CPU implementation timed on a bounded sample of
-the same workload on this machine's host cores.
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline] [--no-plain]
 """
 import argparse
+import glob
 import json
 import math
 import os
-import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-H, N, EPS, TOP = 0.7, 139, 1e-12, 2
+EPS = 1e-12
+CFG = [(1.0, 97), (0.8, 121), (0.7, 139)]
+SLICES = 10
 PEAK_FP64 = 78.6          # TFLOP/s, MI355X FP64 vector (FMA counted as 2)
 KIND = {0: 'level_eval', 1: 'level_dd', 2: 'level_sum', 3: 'nodes_eval', 4: 'nodes_dd', 5: 'level_eval_skip',
         6: 'level_split'}
 KNAME = {'level_eval': 'k_eval<%s, list, AB=1>', 'level_eval_skip': 'k_eval<%s, list, AB=0>'}
-# bounded CPU sample: graphs of the same workload whose reference run takes ~10-30 s on a few cores
-CPU_SAMPLE = ['g3_01', 'g3_07', 'g3_09', 'g3_17', 'g3_19', 'g3_25', 'g3_27', 'v3_00', 'v3_03', 'v3_04', 'v3_06',
-              'v3_07', 'v3_p4Sc', 'v3_p4Sm']
+# bounded samples of the workload: the plain GPU path (the reference's host driver is bound by its serial
+# sums, ~2.5e8 points/s on the largest jobs) and the reference CPU build.  Both hold a D = 7 integral
+# (D = 7 jobs are 91% of the workload's points); the plain sample also the light-by-light integral, whose
+# smallest job here (3.4e9 points of ~15k FP64 operations) takes the CPU build several minutes.
+PLAIN_SAMPLE = [('g3_00', 1.0), ('g3_lbl', 1.0), ('g3_01', 0.7), ('v3_03', 0.8), ('v3_p4Sc', 1.0),
+                ('g2_00', 1.0), ('g2_01', 1.0), ('g2_02', 1.0), ('g2_03', 1.0), ('v2_00', 1.0)]
+CPU_SAMPLE = [j for j in PLAIN_SAMPLE if j[0] != 'g3_lbl']
 
 
 def percentile(xs, p):
@@ -49,47 +64,89 @@ def percentile(xs, p):
     return xs[f] + (xs[c] - xs[f])*(k - f)
 
 
-def run_pass(S, graphs, lat, out=None):
-    """the plain per-graph path: one hierarchical sum after another on the library's stream"""
-    ev = 0
-    for gi, name in graphs:
-        t0 = time.perf_counter()
-        r = S.hsum(gi, H, N, EPS, TOP)
-        lat.append(time.perf_counter() - t0)
-        ev += r['nev']
-        if out is not None:
-            out[name] = r
-    return ev
+def workload(S):
+    """[(graph index, name, D, h, N, top)] of the 138 jobs, and their ten slices"""
+    info = S.graphs()
+    gs = S.select('all2', 'all3')
+    jobs = [(gi, info[gi]['name'], info[gi]['D'], h, N, S.default_top(h)) for h, N in CFG for gi in gs]
+    order = sorted(jobs, key=lambda j: (-j[2], j[3], j[1]))
+    return jobs, [order[i::SLICES] for i in range(SLICES)]
 
 
-def run_batched(S, graphs, lat, out=None):
-    """the batched path: all jobs submitted at once to the engine (continuous batching over its workers);
-    a job's latency is submit -> result"""
+def run_batched(S, steps, lat, out):
+    """the batched path: every job of the given steps submitted to the engine in step order (widest first
+    within a step), then waited for; a job's latency is submit -> result"""
     ev = 0
-    t0 = time.perf_counter()
-    jobs = [(S.submit(gi, H, N, EPS, TOP), name) for gi, name in graphs]
-    for j, name in jobs:
+    ids = [(S.submit(gi, h, N, EPS, top), (name, h)) for step in steps for gi, name, D, h, N, top in step]
+    for j, key in ids:
         r, tq, tr = S.wait(j)
         lat.append(tq + tr)
         ev += r['nev']
-        if out is not None:
-            out[name] = r
+        out.setdefault(key, r)
     return ev
 
 
-def cpu_baseline():
-    """the reference implementation (oracle/_ref/sincqed_ref, OpenMP on all host cores) on a bounded
-    sample of the workload; symmetry fold off as on the device"""
+def run_per_graph(S, jobs, lat, out):
+    """the plain per-graph path: one hierarchical sum after another on the library's stream"""
+    ev = 0
+    for gi, name, D, h, N, top in jobs:
+        t0 = time.perf_counter()
+        r = S.hsum(gi, h, N, EPS, top)
+        lat.append(time.perf_counter() - t0)
+        ev += r['nev']
+        out[(name, h)] = r
+    return ev
+
+
+def run_reference_sample(sample, gpu, cores):
+    """the reference driver (GPU build over this library, or its CPU build) on the sample, one run per h"""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as O
-    if not os.path.exists(O.SINCQED_REF):
+    binary = O.SINCQED_REF_GPU if gpu else O.SINCQED_REF
+    if not os.path.exists(binary):
         return None
-    cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-    ref, wall, _ = O.run_reference(H, N, EPS, CPU_SAMPLE, top=TOP, env={'SINCQED_SYM': '0'}, threads=cores, timeout=900)
-    ev = sum(int(r['nev']) for r in ref.values())
-    return {'value': ev/wall, 'unit': 'points/s', 'cores': cores, 'kind': 'reference',
-            'sample': '%d of the 41 graphs (%s) at the same h, N, eps, top: %d evaluations in %.1f s' % (
-                len(CPU_SAMPLE), ' '.join(CPU_SAMPLE), ev, wall)}
+    ev, wall, recs = 0, 0.0, {}
+    for h, N in CFG:
+        names = [g for g, hh in sample if hh == h]
+        if not names:
+            continue
+        ref, w, out = O.run_reference(h, N, EPS, names, env={'SINCQED_SYM': '0'}, threads=cores, timeout=900, gpu=gpu)
+        if gpu and '# gpu off' in out:
+            raise RuntimeError('the reference GPU build did not reach the device')
+        ev += sum(int(r['nev']) for r in ref.values())
+        wall += w
+        recs.update({(g, h): r for g, r in ref.items()})
+    return ev, wall, recs
+
+
+def fixtures():
+    """the reference's committed results without FP contraction: {(graph, h): record} at eps 1e-12"""
+    out = {}
+    for p in glob.glob(os.path.join(ROOT, 'tests', 'golden', 'fixtures', 'nc_h*_eps1e-12.txt')):
+        h = float(os.path.basename(p).split('_')[1][1:])
+        for line in open(p):
+            if line.startswith('#R '):
+                t = line.split()
+                d = {t[i]: t[i + 1] for i in range(2, len(t) - 1, 2)}
+                out[(t[1], h)] = d
+    return out
+
+
+def check_reference(res, fx):
+    n = bad = 0
+    worst = 0.0
+    for key, r in res.items():
+        f = fx.get(key)
+        if f is None:
+            continue
+        n += 1
+        same = (r['nev'] == int(f['nev']) and r['nq'] == int(f['nq']) and r['kept_top'] == int(f['kept']))
+        rel = abs(r['S'][0] - float(f['S0']))/abs(float(f['S0'])) if float(f['S0']) else abs(r['S'][0])
+        worst = max(worst, rel)
+        bad += not (same and rel <= 1e-12)
+    return {'jobs_checked': n, 'jobs_total': len(res), 'all_match': n > 0 and bad == 0, 'mismatches': bad,
+            'max_rel_value_diff': worst, 'against': 'tests/golden/fixtures/nc_*_eps1e-12.txt (reference built '
+            'without FP contraction, SINCQED_SYM=0): evaluation, fallback and kept counts exact, S_h within 1e-12'}
 
 
 def roofline(S, opc):
@@ -111,25 +168,34 @@ def roofline(S, opc):
             return e['skipped']*oc['ops0'] + (e['points'] - e['skipped'])*oc['ops1']
         return None
     fl = flops(top)
-    rl = None
-    if fl is not None:
-        avg_s = top['ms']/top['launches']*1e-3
-        ach = fl/top['launches']/avg_s/1e12
-        rl = {'bound': 'fp64-valu', 'kernel': KNAME[top['kind']] % top['graph'] if top['kind'] in KNAME else top['kind'],
-              'achieved': round(ach, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s', 'frac': round(ach/PEAK_FP64, 4),
-              'traffic': None, 'launches': top['launches'], 'avg_launch_ms': round(top['ms']/top['launches'], 4),
-              'flops_per_launch': fl/top['launches'], 'share_of_device_time': round(top['ms']/tot_ms, 3)}
-    if rl:
-        # HBM bytes per launch of this kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-        # (each counter scaled by what it reads for the library's copy kernel of known size)
-        import glob
-        for p in glob.glob(os.path.join(ROOT, 'profiles', 'traffic_*.json')):
-            t = json.load(open(p))
-            if t.get('kernel') == rl['kernel']:
-                rl['traffic'] = t['traffic_bytes_per_dispatch']
-                rl['traffic_source'] = os.path.relpath(p, ROOT)
+    if fl is None:
+        return None, ent
+    avg_s = top['ms']/top['launches']*1e-3
+    ach = fl/top['launches']/avg_s/1e12
+    kname = KNAME[top['kind']] % top['graph'] if top['kind'] in KNAME else top['kind']
+    rl = {'bound': 'fp64-valu', 'kernel': kname, 'achieved': round(ach, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
+          'frac': round(ach/PEAK_FP64, 4), 'traffic': None, 'rate': 'algorithmic: the source operations of the '
+          'integrand (+ - * /, SURVEY.md Appendix A counting) x valid points per launch / mean launch time',
+          'launches': top['launches'], 'avg_launch_ms': round(top['ms']/top['launches'], 4),
+          'points_per_launch': top['points']/top['launches'], 'flops_per_launch': fl/top['launches'],
+          'share_of_device_time': round(top['ms']/tot_ms, 3)}
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each counter scaled
+    # by what it reads for the library's copy kernel of known size); the executed FP64 work per launch from
+    # the committed SQ_INSTS_VALU_*_F64 counters: 64 lanes x (ADD + MUL + TRANS + 2 FMA)
+    for p in glob.glob(os.path.join(ROOT, 'profiles', 'traffic_*.json')):
+        t = json.load(open(p))
+        if t.get('kernel') == kname:
+            rl['traffic'] = t['traffic_bytes_per_dispatch']
+            rl['traffic_source'] = os.path.relpath(p, ROOT)
+    for p in glob.glob(os.path.join(ROOT, 'profiles', 'valu_*.json')):
+        t = json.load(open(p))
+        if t.get('kernel') == kname:
+            ex = t['executed_fp64_flops_per_dispatch']
+            rl['executed'] = {'achieved': round(ex/avg_s/1e12, 3), 'frac': round(ex/avg_s/1e12/PEAK_FP64, 4),
+                              'flops_per_launch': ex, 'fp64_issue_busy': t.get('fp64_issue_busy'),
+                              'source': os.path.relpath(p, ROOT)}
     agg = [e for e in ent if e['kind'] in ('level_eval', 'level_eval_skip')]
-    if agg and rl:
+    if agg:
         f_all = sum(flops(e) for e in agg)
         ms_all = sum(e['ms'] for e in agg)
         rl['all_level_eval'] = {'achieved': round(f_all/(ms_all*1e-3)/1e12, 3), 'ms': round(ms_all, 1),
@@ -140,82 +206,127 @@ def roofline(S, opc):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1)
-    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--graphs', default='all3', help='selector (default all3); a subset is not the benchmark')
-    ap.add_argument('--profile-out', default=None, help='write the per-kernel table as JSON here')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU path sample')
     ap.add_argument('--workers', type=int, default=16, help='engine workers of the batched path')
-    ap.add_argument('--no-batched', action='store_true',
-                    help='run the per-graph pass only (one host thread: the form profiled with rocprofv3)')
+    ap.add_argument('--profile-only', action='store_true',
+                    help='only the profiled per-graph pass (one host thread: the form profiled with rocprofv3)')
+    ap.add_argument('--graphs', default='all3', help='graphs of the profiled pass (at h = 0.7)')
+    ap.add_argument('--profile-out', default=None, help='write the per-kernel table of the profiled pass as JSON here')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
     a = ap.parse_args()
     if a.gpus != 1:
-        print('bench.py: this build measures one GPU; the Sinc sum of one graph is not split across GPUs here '
+        print('bench.py: this build measures one GPU; the Sinc sums are not split across GPUs here '
               '(--gpus %d is not supported)' % a.gpus, file=sys.stderr)
         sys.exit(2)
     from sincfeyn_amd import sfx as S
     info = S.init(0)
     opc = json.load(open(os.path.join(ROOT, 'sincfeyn_amd', '_gen', 'opcounts.json')))
-    graphs = [(i, S.graphs()[i]['name']) for i in S.select(*a.graphs.split(','))]
     if a.copy_probe:
         ms = S.C.c_double()
         S._check(S.lib().sfx_copy_probe(S.C.c_long(a.copy_probe), S.C.byref(ms)))
         print('copy probe: %d bytes in %.3f ms' % (a.copy_probe, ms.value), file=sys.stderr)
-    # the plain per-graph path, profiled (kernel durations from HIP events on the library's stream)
-    lat_pg = []
-    for _ in range(a.warmup):
-        run_pass(S, graphs, [])
-    S._check(S.lib().sfx_sync())
+
+    def log(*x):
+        print('[bench]', *x, file=sys.stderr, flush=True)
+
+    # the profiled per-graph pass (kernel durations from HIP events on the library's stream)
+    pgraphs = [(gi, S.graphs()[gi]['name'], S.graphs()[gi]['D'], 0.7, 139, 2) for gi in S.select(*a.graphs.split(','))]
+    S.hsum(pgraphs[0][0], 0.7, 139, EPS, 2)                         # first launches of the kernels
     S._check(S.lib().sfx_prof_enable(1))
     t0 = time.perf_counter()
-    ev_pg = 0
-    per_graph = {}
-    for _ in range(a.steps):
-        ev_pg += run_pass(S, graphs, lat_pg, per_graph)
+    ev_p = run_per_graph(S, pgraphs, [], {})
     S._check(S.lib().sfx_sync())
-    dt_pg = time.perf_counter() - t0
+    dt_p = time.perf_counter() - t0
     S._check(S.lib().sfx_prof_enable(0))
     rl, ent = roofline(S, opc)
     if a.profile_out:
         with open(a.profile_out, 'w') as f:
             json.dump(ent, f, indent=1)
-    if a.no_batched:
-        print(json.dumps({'metric': 'Sinc-sum points/s, per-graph pass only (profiling form)', 'value': ev_pg/dt_pg,
-                          'unit': 'points/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
-                          'ms_per_step': dt_pg*1e3/a.steps, 'higher_is_better': True, 'path': 'per-graph',
-                          'roofline': rl, 'device': info}))
+    log('profiled per-graph pass: %d points in %.2f s' % (ev_p, dt_p))
+    if a.profile_only:
+        print(json.dumps({'metric': 'Sinc-sum points/s, profiled per-graph pass only', 'value': ev_p/dt_p,
+                          'unit': 'points/s', 'n_gpus': 1, 'path': 'per-graph', 'roofline': rl, 'device': info}))
         return
-    # the batched path: the same jobs through the engine (headline value)
+
+    jobs, slices = workload(S)
+    steps = [slices[i % SLICES] for i in range(a.warmup + a.steps)]
     S.engine_start(a.workers, 0)
-    order = sorted(graphs, key=lambda g: -S.graphs()[g[0]]['D'])      # the widest lattices first
-    for _ in range(a.warmup):
-        run_batched(S, order, [])
-    lat = []
-    batched = {}
+    run_batched(S, steps[:a.warmup], [], {})
+    lat, batched = [], {}
     t0 = time.perf_counter()
-    ev = 0
-    for _ in range(a.steps):
-        ev += run_batched(S, order, lat, batched)
+    ev = run_batched(S, steps[a.warmup:], lat, batched)
     dt = time.perf_counter() - t0
-    same = all(batched[n]['S'] == per_graph[n]['S'] and batched[n]['nev'] == per_graph[n]['nev'] and
-               batched[n]['nq'] == per_graph[n]['nq'] for _, n in graphs)
-    cpu = None if a.no_cpu_baseline else cpu_baseline()
+    log('batched: %d steps, %d points in %.2f s' % (a.steps, ev, dt))
+    # the plain per-graph path over the same 138 jobs, without profiling
+    lat_pg, per_graph = [], {}
+    t0 = time.perf_counter()
+    ev_pg = run_per_graph(S, jobs, lat_pg, per_graph)
+    dt_pg = time.perf_counter() - t0
+    log('per-graph: %d points in %.2f s' % (ev_pg, dt_pg))
+    same = all(r['S'] == per_graph[k]['S'] and r['nev'] == per_graph[k]['nev'] and r['nq'] == per_graph[k]['nq']
+               for k, r in batched.items())
+    ref = check_reference(batched, fixtures())
+
+    plain = None
+    cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    if not a.no_plain:
+        got = run_reference_sample(PLAIN_SAMPLE, True, cores)
+        if got:
+            ev_s, w_s, recs = got
+            sjobs = [j for j in jobs if (j[1], j[3]) in set(PLAIN_SAMPLE)]
+            t0 = time.perf_counter()
+            ev_sb = run_batched(S, [sjobs], [], {})
+            dt_sb = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            samp_pg = {}
+            ev_sp = run_per_graph(S, sjobs, [], samp_pg)
+            dt_sp = time.perf_counter() - t0
+            agree = all(r['nev'] == samp_pg[k]['nev'] and r['nq'] == samp_pg[k]['nq'] and r['kept'] == samp_pg[k]['kept_top']
+                        and abs(r['S0'] - samp_pg[k]['S'][0]) <= 1e-12*abs(r['S0']) for k, r in recs.items())
+            plain = {'value': ev_s/w_s, 'unit': 'points/s', 'host_threads': cores, 'wall_s': w_s, 'points': ev_s,
+                     'sample': ' '.join('%s@h%g' % j for j in PLAIN_SAMPLE) + ' (%.1f%% of the workload\'s points)' % (
+                         100.0*ev_s/ev*a.steps/SLICES if ev else 0),
+                     'path': 'oracle/_ref/sincqed_ref_gpu: the reference driver (-DSQ_GPU) over src/sfx_gpu.c',
+                     'batched_on_sample': ev_sb/dt_sb, 'per_graph_on_sample': ev_sp/dt_sp,
+                     'batched_over_plain': (ev_sb/dt_sb)/(ev_s/w_s),
+                     'results_agree_with_device': agree}
+            log('plain GPU sample: %d points in %.2f s' % (ev_s, w_s))
+    cpu = None
+    if not a.no_cpu_baseline:
+        got = run_reference_sample(CPU_SAMPLE, False, cores)
+        if got:
+            ev_c, w_c, _ = got
+            cpu = {'value': ev_c/w_c, 'unit': 'points/s', 'cores': cores, 'kind': 'reference',
+                   'sample': '%s at eps 1e-12: %d evaluations in %.1f s (%.2f%% of the workload\'s points; one D = 7 '
+                             'integral; g3_lbl left out: its smallest job here takes minutes on the CPU)' % (
+                                 ' '.join('%s@h%g' % j for j in CPU_SAMPLE), ev_c, w_c, 100.0*ev_c/ev*a.steps/SLICES if ev else 0)}
+            log('cpu sample: %d points in %.2f s' % (ev_c, w_c))
     out = {
         'metric': 'Sinc-sum points/s batched vs per-graph; p50/p99 job latency at <=1e-12 rel',
         'value': ev/dt, 'unit': 'points/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
         'ms_per_step': dt*1e3/a.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f64', 'data': 'deterministic Sinc lattice of the named integrals (no input data)',
-        'config': {'workload': 'all41 three-loop integrals (h=0.7, N=139, eps=1e-12), batched through the job engine',
-                   'graphs': len(graphs), 'h': H, 'N': N, 'K': S.lattice_K(N, TOP), 'eps': EPS, 'top': TOP,
-                   'points_per_step': ev//a.steps, 'path': 'batched', 'workers': a.workers},
-        'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat)},
-        'per_graph': {'value': ev_pg/dt_pg, 'ms_per_step': dt_pg*1e3/a.steps,
+        'config': {'workload': 'batched heterogeneous launch: all2 + all3 (46 integrals) x h in {1.0, 0.8, 0.7} '
+                               '(N 97/121/139, top 1/1/2), eps 1e-12: 138 jobs through the job engine; a step is one '
+                               'of ten fixed slices of the jobs (widest first, dealt round-robin)',
+                   'jobs': len(jobs), 'jobs_per_step': len(slices[0]), 'eps': EPS, 'points_per_step': ev//a.steps,
+                   'path': 'batched', 'workers': a.workers, 'precision': 'FP64 bulk, double-double where flagged'},
+        'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat),
+                      'note': 'submit -> result with every timed step queued at once'},
+        'per_graph': {'value': ev_pg/dt_pg, 'wall_s': dt_pg, 'jobs': len(jobs),
                       'latency_s': {'p50': percentile(lat_pg, 50), 'p99': percentile(lat_pg, 99)}},
         'batched_over_per_graph': (ev/dt)/(ev_pg/dt_pg),
         'batched_results_identical_to_per_graph': same,
+        'reference_check': ref,
+        'plain_gpu': plain,
+        'batched_over_plain_gpu': (ev/dt)/plain['value'] if plain else None,
         'roofline': rl,
+        'roofline_pass': {'jobs': '%d integrals (%s) at h 0.7, N 139, eps 1e-12, per graph' % (len(pgraphs), a.graphs),
+                          'points': ev_p, 'wall_s': dt_p},
         'cpu_baseline': cpu,
         'device': info,
     }

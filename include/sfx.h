@@ -16,6 +16,8 @@
  *                                graphs.c:8-33) and its groups (graphs.c:15-29)
  *   sfx_eval_nodes / _cells      gpu_submit + gpu_wait, and gpu_submit_cells  (driver/gpu.h:17-20,
  *                                gpu_kernel.h:22-36: f w and Sum|terms| w per node, mode 0 / mode 1)
+ *   sfx_slot_*                   the same, asynchronous in two slots: gpu_setup / gpu_submit /
+ *                                gpu_submit_cells / gpu_wait (driver/gpu.h:12-22, gpu.cu:54-103)
  *   sfx_eval_raw                 the kernels' fd / fq at given s (common.h:27-29), golden primitive
  *   sfx_hsum                     hsum() with lattice_setup()                  (driver/sum.c:556-814,
  *                                lattice.c:13-27), symmetry fold off as in the reference GPU build (sum.c:571)
@@ -142,6 +144,24 @@ int sfx_eval_nodes(int graph, double h, int K, long n, const int16_t *k, const u
                    double *fw, double *abw);
 int sfx_eval_cells(int graph, double h, int K, long nc, const int16_t *cells, const uint8_t *skip, int st,
                    double *fw, double *abw);
+
+/* ---- the reference's GPU back end, asynchronous, in two slots (driver/gpu.h:12-22, gpu.cu:54-103) ----
+ * For a host driver that keeps its own lattice loops, sums and double-double fallback and sends batches to
+ * the device two at a time (the reference's driver built with SQ_GPU; src/sfx_gpu.c binds these to its
+ * gpu_setup / gpu_submit / gpu_submit_cells / gpu_wait).  Called from one host thread.
+ * sfx_slot_setup: the graph named `name` (with D lines) and the caller's table EX[k+K] = e^{kh},
+ *   uploaded as given; returns 1, or 0 if this build has no device kernel for the graph (the caller then
+ *   evaluates on the CPU), or a negative SFX_E* code.  Waits for both slots.
+ * sfx_slot_submit: n nodes k[i*stride + d] (|k| <= K), skip[i] != 0 evaluates node i without the |.| sums
+ *   (skip may be NULL); copies, launches and copies back, returning at once.
+ * sfx_slot_submit_cells: the 2^D children k = cell - st b of nc cells (cells[c*stride + d]), child b of cell c
+ *   at (c << D) | b, skip per cell; a child below the box gives 0, 0 (gpu_kernel.h:22-36, mode 1).
+ * sfx_slot_wait: waits for the slot; fw[i] = f w, abw[i] = Sum|terms| w (0 where skipped), in pinned host
+ *   memory valid until the slot's next submit. */
+int sfx_slot_setup(const char *name, int D, const double *EX, int K, double H);
+int sfx_slot_submit(int slot, long n, const int32_t *k, int stride, const uint8_t *skip);
+int sfx_slot_submit_cells(int slot, long nc, const int32_t *cells, int stride, const uint8_t *skip, int st);
+int sfx_slot_wait(int slot, const double **fw, const double **abw);
 
 /* Scale test of a graph's integrand: replaces the reference's `sincqed scale` (driver/main.c:77-116,
  * scale_test).  From a point x0 (srand(seed); x0_i = exp(4 rand()/RAND_MAX - 2); x0_fixed = 1) every

@@ -18,6 +18,7 @@ LIB_KERNELS_NC = os.path.join(REF_DIR, 'libsqk_nc.so')
 LIB_KERNELS_F128 = os.path.join(REF_DIR, 'libsqk_f128.so')
 SINCQED_REF = os.path.join(REF_DIR, 'sincqed_ref')
 SINCQED_REF_NC = os.path.join(REF_DIR, 'sincqed_ref_nc')
+SINCQED_REF_GPU = os.path.join(REF_DIR, 'sincqed_ref_gpu')   # the reference driver's GPU build over libsfx
 
 
 class OrcOpts(C.Structure):
@@ -167,16 +168,17 @@ def default_top(h, topstep=3.0):
 _RLINE = re.compile(r'^#R (\S+) (.*)$')
 
 
-def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=600, nc=False):
+def run_reference(h, N, eps, graphs, top=None, env=None, threads=None, timeout=600, nc=False, gpu=False):
     """the reference implementation (oracle/_ref/sincqed_ref, or sincqed_ref_nc built without FP
-    contraction) -> ({graph: {field: value}}, wall seconds, stdout)"""
+    contraction, or with gpu sincqed_ref_gpu: that driver's GPU build over this library's gpu.h back end)
+    -> ({graph: {field: value}}, wall seconds, stdout)"""
     e = dict(os.environ)
     e.update({'SINCQED_CKPT': '0', 'SINCQED_PROGRESS': '0', 'SINCQED_RESULTS': '-'})
     if threads:
         e['OMP_NUM_THREADS'] = str(threads)
     if env:
         e.update(env)
-    args = [SINCQED_REF_NC if nc else SINCQED_REF, 'run', repr(float(h)), str(N), repr(float(eps))]
+    args = [SINCQED_REF_GPU if gpu else SINCQED_REF_NC if nc else SINCQED_REF, 'run', repr(float(h)), str(N), repr(float(eps))]
     if top is not None:
         args.append(str(top))
     args += list(graphs)

@@ -13,6 +13,9 @@
  */
 #include "sq.h"
 #include <dlfcn.h>
+#ifdef SQ_GPU
+#include "gpu.h"
+#endif
 
 #ifndef SQ_PREFIX
 #define SQ_PREFIX "G_"          /* descriptor symbol prefix of the linked kernels */
@@ -107,6 +110,9 @@ int main(int argc, char **argv){
     for(int j = 0; j < ns; j++){ CURI = j + 1; run_graph(sel[j], h, N, eps, top, &tot, &toterr, &toterr2); }
     if(ns > 1 && RESF) fprintf(RESF, "# SUM %d graphs  %+.15e  err %.3e (linear)  %.3e (quadrature)\n", ns, tot, toterr, sqrt(toterr2));
     printf("# wall %.3f s\n", wtime() - T0);
+#ifdef SQ_GPU
+    printf("# gpu %s\n", gpu_info());          /* "off" when no graph ran on the device */
+#endif
     if(RESF) fclose(RESF);
     return 0;
 }

@@ -191,6 +191,17 @@ def build_cli():
         raise RuntimeError('sincqed CLI build failed:\n%s' % p.stderr[-4000:])
     os.replace(CLI + '.tmp', CLI)
     log('built %s' % os.path.relpath(CLI, ROOT))
+    # sincFeyn's GPU back-end ABI (driver/gpu.h) over the library: what its host driver links instead of
+    # driver/gpu.cu
+    gsrc, glib = os.path.join(ROOT, 'src', 'sfx_gpu.c'), os.path.join(LIBDIR, 'libsfxgpu.so')
+    if os.path.exists(gsrc):
+        cmd = ['gcc', '-O2', '-std=c11', '-Wall', '-fPIC', '-shared', '-I' + os.path.join(ROOT, 'include'), '-o', glib + '.tmp',
+               gsrc, '-L' + LIBDIR, '-lsfx', '-Wl,-rpath,$ORIGIN']
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError('libsfxgpu build failed:\n%s' % p.stderr[-4000:])
+        os.replace(glib + '.tmp', glib)
+        log('built %s' % os.path.relpath(glib, ROOT))
     return CLI
 
 

@@ -24,6 +24,10 @@ typedef struct {
     double ddslack;          /* DDSLACK (eval.c:17) */
     int dropunres;           /* DROPUNRES (eval.c:24) */
     int abcheck;             /* ABCHECK: skip-class points evaluated and tested in full (eval.c:11-12, 35) */
+    double *raw_ab;          /* device, or NULL.  Set: the evaluation applies no test -- it writes f w (untested)
+                                as the value and Sum|terms| w here, at the point's slot (0 where skipped or
+                                outside the box): the outputs of the reference's GPU kernel (gpu_kernel.h:22-36),
+                                whose caller applies eval.c's tests on the host */
 } sfx_eval_params_t;
 
 /* Per-child flag byte written by the evaluation kernels. */

@@ -133,7 +133,7 @@ struct Neu {
 
 struct Job {
     const sfx_graph_entry *g; int gi, D, K, top; double h, eps;
-    sfx_opts_t o; sfx_eval_params_t P;
+    sfx_opts_t o; sfx_eval_params_t P{};
     hipStream_t s; Ctx *ctx;
     JobProgress *pg = nullptr;         // where the sum is (stage, level, block), for the engine's messages
 };
@@ -228,6 +228,7 @@ static int box_zero(Job &J, int lev, int *zero, long *nev, long *unresdrop){
 static void set_params(Job &J, double abserr){
     J.P.K = J.K; J.P.H = J.h; J.P.abskip = J.o.abskip; J.P.abserr = abserr; J.P.nearf = J.o.nearf;
     J.P.ddslack = J.o.ddslack; J.P.dropunres = J.o.dropunres; J.P.abcheck = J.o.abcheck; J.P.EX = J.ctx->B.EX.as<double>();
+    J.P.raw_ab = nullptr;                       // the per-point tests in the kernel
 }
 
 // grow a device buffer keeping its first `used` bytes (DevBuf::reserve does not preserve contents)
@@ -661,3 +662,5 @@ extern "C" int sfx_job_wait_for(long job, double timeout_s, sfx_result_t *res, d
 }
 
 extern "C" int sfx_job_release(long job){ return engine_release(job); }
+
+#include "sfx_slots.inc"

@@ -114,7 +114,7 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
     // full (eval.c:35, 57) and the launch for that class uses AB = 1
     const bool cls = valid && P.abskip && (MODE == MODE_NODES ? cfl != 0u : (cfl & 1u) != 0u);
     bool skip = cls && !P.abcheck;
-    double v = 0; unsigned fl = 0;
+    double v = 0, rab = 0; unsigned fl = 0;
     if(valid){
         double sf[D]; double w = lattice_node<D>(sEX, P.K, P.H, k, sf);
         double ab = 0, f;
@@ -126,22 +126,27 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
             else f = G::template f<double>(sf, &ab, 1);
         }
         v = f*w; fl = SFX_FL_VALID | (MODE == MODE_NODES && cls ? SFX_FL_SKIPCLS : 0u);
-        if(MODE == MODE_NODES && st > 0){
-            // the node's outer-shell class at step st (the grow's and the box scan's e1/e2 weights)
-            int mx = 0;
+        if(P.raw_ab){
+            // the reference GPU kernel's outputs, untested: f w, and Sum|terms| w unless skipped
+            rab = skip ? 0.0 : ab*w; fl = SFX_FL_VALID;
+        } else {
+            if(MODE == MODE_NODES && st > 0){
+                // the node's outer-shell class at step st (the grow's and the box scan's e1/e2 weights)
+                int mx = 0;
 #pragma unroll
-            for(int d = 0; d < D; d++){ int a = k[d] < 0 ? -k[d] : k[d]; mx = a > mx ? a : mx; }
-            fl |= mx == P.K ? SFX_FL_SHELL1 : mx == P.K - st ? SFX_FL_SHELL2 : 0u;
-        }
-        if(skip){ fl |= SFX_FL_SKIP; if(!isfinite(v)) v = 0; }
-        else {
-            double bnd = ab*w*16*DBL_EPSILON/P.ddslack;            // eval.c:36, same association
-            if(!(bnd <= P.nearf*P.abserr)) fl |= SFX_FL_NEAR;
-            if(!(bnd <= P.abserr)){ fl |= SFX_FL_DD; v = 0; }
-            else if(!isfinite(v)) v = 0;
+                for(int d = 0; d < D; d++){ int a = k[d] < 0 ? -k[d] : k[d]; mx = a > mx ? a : mx; }
+                fl |= mx == P.K ? SFX_FL_SHELL1 : mx == P.K - st ? SFX_FL_SHELL2 : 0u;
+            }
+            if(skip){ fl |= SFX_FL_SKIP; if(!isfinite(v)) v = 0; }
+            else {
+                double bnd = ab*w*16*DBL_EPSILON/P.ddslack;            // eval.c:36, same association
+                if(!(bnd <= P.nearf*P.abserr)) fl |= SFX_FL_NEAR;
+                if(!(bnd <= P.abserr)){ fl |= SFX_FL_DD; v = 0; }
+                else if(!isfinite(v)) v = 0;
+            }
         }
     }
-    if(in){ vout[o] = v; flout[o] = (uint8_t)fl; }
+    if(in){ vout[o] = v; flout[o] = (uint8_t)fl; if(P.raw_ab) P.raw_ab[o] = rab; }
     if(ddidx){
         // the points that need the double-double recompute, appended to ddidx (one atomic per wave; the
         // list's order is immaterial: the recompute writes each point at its own slot)

@@ -192,6 +192,48 @@ def eval_cells(graph, h, K, cells, st, skip=None):
     return fw, abw
 
 
+def _slot_sigs():
+    L = lib()
+    if not getattr(L, '_slot_sigs', False):
+        L.sfx_slot_setup.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_int, C.c_double]
+        L.sfx_slot_submit.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_int, C.c_void_p]
+        L.sfx_slot_submit_cells.argtypes = [C.c_int, C.c_long, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.sfx_slot_wait.argtypes = [C.c_int, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double))]
+        L._slot_sigs = True
+    return L
+
+
+def slot_setup(name, D, EX, K, H):
+    """the reference back end's gpu_setup (driver/gpu.h:17): 1 if the graph has a device kernel, else 0"""
+    EX = np.ascontiguousarray(EX, dtype=np.float64)
+    rc = _slot_sigs().sfx_slot_setup(name.encode(), D, EX.ctypes.data, K, H)
+    if rc < 0:
+        _check(rc)
+    return rc
+
+
+def slot_submit(slot, k, skip=None, cells_step=0):
+    """gpu_submit (nodes k[n, stride] int32) or, with cells_step > 0, gpu_submit_cells; returns at once.
+    The arrays must stay alive until slot_wait (the library copies them before returning, so they may go)"""
+    k = np.ascontiguousarray(k, dtype=np.int32)
+    n, stride = k.shape
+    sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.uint8)
+    L = _slot_sigs()
+    if cells_step:
+        _check(L.sfx_slot_submit_cells(slot, n, k.ctypes.data, stride, None if sk is None else sk.ctypes.data, cells_step))
+    else:
+        _check(L.sfx_slot_submit(slot, n, k.ctypes.data, stride, None if sk is None else sk.ctypes.data))
+
+
+def slot_wait(slot, n):
+    """gpu_wait: copies of (f w, Sum|terms| w) of the slot's last batch of n points"""
+    fw = C.POINTER(C.c_double)(); abw = C.POINTER(C.c_double)()
+    _check(_slot_sigs().sfx_slot_wait(slot, C.byref(fw), C.byref(abw)))
+    if n == 0:
+        return np.zeros(0), np.zeros(0)
+    return np.ctypeslib.as_array(fw, (n,)).copy(), np.ctypeslib.as_array(abw, (n,)).copy()
+
+
 QD_OPS = {'+': 0, '-': 1, '*': 2, '/': 3}
 
 
