@@ -213,7 +213,8 @@ def main():
     ap.add_argument('--workers', type=int, default=16, help='engine workers of the batched path')
     ap.add_argument('--profile-only', action='store_true',
                     help='only the profiled per-graph pass (one host thread: the form profiled with rocprofv3)')
-    ap.add_argument('--graphs', default='all3', help='graphs of the profiled pass (at h = 0.7)')
+    ap.add_argument('--graphs', default='all3', help='graphs of the profiled pass')
+    ap.add_argument('--profile-h', default='0.7', help='h values (comma-separated) of the profiled pass')
     ap.add_argument('--profile-out', default=None, help='write the per-kernel table of the profiled pass as JSON here')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
@@ -234,8 +235,9 @@ def main():
         print('[bench]', *x, file=sys.stderr, flush=True)
 
     # the profiled per-graph pass (kernel durations from HIP events on the library's stream)
-    pgraphs = [(gi, S.graphs()[gi]['name'], S.graphs()[gi]['D'], 0.7, 139, 2) for gi in S.select(*a.graphs.split(','))]
-    S.hsum(pgraphs[0][0], 0.7, 139, EPS, 2)                         # first launches of the kernels
+    pgraphs = [(gi, S.graphs()[gi]['name'], S.graphs()[gi]['D'], h, N, S.default_top(h))
+               for h, N in CFG if repr(h) in a.profile_h.split(',') for gi in S.select(*a.graphs.split(','))]
+    S.hsum(pgraphs[0][0], pgraphs[0][3], pgraphs[0][4], EPS, pgraphs[0][5])     # first launches of the kernels
     S._check(S.lib().sfx_prof_enable(1))
     t0 = time.perf_counter()
     ev_p = run_per_graph(S, pgraphs, [], {})
@@ -267,7 +269,9 @@ def main():
     ev_pg = run_per_graph(S, jobs, lat_pg, per_graph)
     dt_pg = time.perf_counter() - t0
     log('per-graph: %d points in %.2f s' % (ev_pg, dt_pg))
-    same = all(r['S'] == per_graph[k]['S'] and r['nev'] == per_graph[k]['nev'] and r['nq'] == per_graph[k]['nq']
+    def seq(a_, b_):        # S_h, S_2h, S_4h bit for bit (NaN where a lattice has no such level)
+        return all(x == y or (math.isnan(x) and math.isnan(y)) for x, y in zip(a_, b_))
+    same = all(seq(r['S'], per_graph[k]['S']) and r['nev'] == per_graph[k]['nev'] and r['nq'] == per_graph[k]['nq']
                for k, r in batched.items())
     ref = check_reference(batched, fixtures())
 
@@ -325,7 +329,7 @@ def main():
         'plain_gpu': plain,
         'batched_over_plain_gpu': (ev/dt)/plain['value'] if plain else None,
         'roofline': rl,
-        'roofline_pass': {'jobs': '%d integrals (%s) at h 0.7, N 139, eps 1e-12, per graph' % (len(pgraphs), a.graphs),
+        'roofline_pass': {'jobs': '%d jobs (%s at h %s), eps 1e-12, per graph' % (len(pgraphs), a.graphs, a.profile_h),
                           'points': ev_p, 'wall_s': dt_p},
         'cpu_baseline': cpu,
         'device': info,

@@ -150,15 +150,17 @@ static inline double *node_v(JobBufs &B){ return B.vfe.as<double>(); }
 static inline double *node_e2(JobBufs &B, long n){ return B.vfe.as<double>() + n; }
 static inline uint8_t *node_fl(JobBufs &B, long n){ return (uint8_t*)(B.vfe.as<double>() + 2*n); }
 
-// nodes already on the device (dpts); the results stay in B.vfe (node_v, node_fl) and are copied to v, fl
-// (raw flag bytes); with st > 0 the flag bytes also carry the outer-shell class of each node at that
-// step.  One host round trip: one copy brings back values, round terms and flags, and the fallback
-// points' round terms and unresolved marks are summed in node order from the flags.
-static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, double *v, uint8_t *fl, long *nq, double *u, long *nd){
+// the host's view of a node list's results: values, round terms (fallback points only) and flag bytes
+// in the context's pinned staging, valid until its next node evaluation
+struct NodeOut { const double *v, *e2; const uint8_t *fl; };
+
+// nodes already on the device (dpts); the results stay in B.vfe (node_v, node_fl) and come back to the
+// pinned staging H.vfe in one copy (with st > 0 the flag bytes also carry the outer-shell class of each
+// node at that step); the caller sums what it needs in node order
+static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeOut *out){
     if(n <= 0) return SFX_OK;
     JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; Ctx &X = *J.ctx; int rc;
-    // at least 4 MiB each, so a job's growing frontiers rarely reallocate (a device free synchronises the
-    // whole device, every worker's stream included)
+    // at least 4 MiB each, so a job's growing frontiers rarely reallocate
     const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
     if((rc = B.vfe.reserve(want)) || (rc = B.ddidx.reserve(std::max<size_t>(n*sizeof(uint32_t), (size_t)1 << 20))) ||
        (rc = B.ddcnt.reserve(64)) || (rc = H.vfe.reserve(want))) return rc;
@@ -174,13 +176,23 @@ static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, doubl
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
     SFX_HIP(hipMemcpyAsync(H.vfe.p, B.vfe.p, bytes, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
-    const double *hv = H.vfe.as<double>(), *e2 = hv + n; const uint8_t *hf = (const uint8_t*)(hv + 2*n);
-    memcpy(v, hv, n*sizeof(double)); memcpy(fl, hf, n);
-    long m = 0;
-    for(long i = 0; i < n; i++) if(fl[i] & SFX_FL_DD){ m++; *u += e2[i]; if(fl[i] & SFX_FL_UNRES) (*nd)++; }
-    if(pon){ prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m); }
-    *nq += m;
+    const double *hv = H.vfe.as<double>();
+    out->v = hv; out->e2 = hv + n; out->fl = (const uint8_t*)(hv + 2*n);
+    if(pon){
+        long m = 0; for(long i = 0; i < n; i++) m += (out->fl[i] & SFX_FL_DD) != 0;
+        prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m);
+    }
     return SFX_OK;
+}
+
+// the fallback points' count, round terms (in node order) and unresolved count of a node list
+static void fallback_sums(const NodeOut &o, long n, long *nq, double *u, long *nd){
+    long m = 0, d = 0; double uu = *u;
+    for(long i = 0; i < n; i++){
+        const bool q = (o.fl[i] & SFX_FL_DD) != 0;
+        m += q; d += q && (o.fl[i] & SFX_FL_UNRES); uu += q ? o.e2[i] : 0.0;     // + 0.0 leaves the sum's bits
+    }
+    *nq += m; *u = uu; *nd += d;
 }
 
 static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v, uint8_t *flg, long *nq, double *u, long *nd){
@@ -189,9 +201,11 @@ static int eval_nodes_finished(Job &J, const sfx_cell_t *pts, long n, double *v,
     if((rc = B.pts.reserve(n*sizeof(sfx_cell_t)))) return rc;
     hipStream_t s = J.s;
     if((rc = upload(*J.ctx, s, B.pts.p, pts, n*sizeof(sfx_cell_t)))) return rc;
-    std::vector<uint8_t> fl(n);
-    if((rc = eval_nodes_core(J, B.pts.as<sfx_cell_t>(), n, 0, v, fl.data(), nq, u, nd))) return rc;
-    for(long i = 0; i < n; i++) flg[i] = (fl[i] & (SFX_FL_NEAR | SFX_FL_DD)) | ((fl[i] & SFX_FL_UNRES) ? SFX_PT_UNRES : 0);
+    NodeOut o;
+    if((rc = eval_nodes_core(J, B.pts.as<sfx_cell_t>(), n, 0, &o))) return rc;
+    fallback_sums(o, n, nq, u, nd);
+    memcpy(v, o.v, n*sizeof(double));
+    for(long i = 0; i < n; i++) flg[i] = (o.fl[i] & (SFX_FL_NEAR | SFX_FL_DD)) | ((o.fl[i] & SFX_FL_UNRES) ? SFX_PT_UNRES : 0);
     return SFX_OK;
 }
 
@@ -202,6 +216,7 @@ struct GrowOut {
     double ref = 0;                 // the reference scale Ref: A of the grow, of the coarser box for the box scans
     // host seconds of the stages (the verbose report, and t_grow / t_dilate of the result)
     double t_loop = 0, t_final = 0, t_tiers = 0, t_dilate = 0; long frontiers = 0, tier_steps = 0;
+    double t_f_eval = 0, t_f_host = 0, t_f_dev = 0;   // the frontier loop: evaluation + copy back, host sums, the rest
     double gshs = 0; long gshn = 0;   // the shell: evaluated points below the running threshold, never expanded
     long nskip = 0, nmiss = 0;        // frontier points of the skip class, and those that needed the fallback
 };
@@ -515,8 +530,9 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         if(!J.o.box_scan && top == 0 && G.gshn) res->prune = fabs(G.gshs);
         double t1 = now(); res->t_grow = t1 - t0 - G.t_dilate; res->t_dilate = G.t_dilate;
         if(J.o.verbose)
-            fprintf(stderr, "  %s coarsest level: %.2f ms (%ld frontiers %.2f ms, final threshold %.2f ms, tiers %ld steps %.2f ms, "
-                    "dilation %.2f ms)\n", J.g->name, 1e3*(t1 - t0), G.frontiers, 1e3*G.t_loop, 1e3*G.t_final, G.tier_steps,
+            fprintf(stderr, "  %s coarsest level: %.2f ms (%ld frontiers %.2f ms [evaluation and copy back %.2f, host sums %.2f, "
+                    "candidates %.2f], final threshold %.2f ms, tiers %ld steps %.2f ms, dilation %.2f ms)\n", J.g->name, 1e3*(t1 - t0),
+                    G.frontiers, 1e3*G.t_loop, 1e3*G.t_f_eval, 1e3*G.t_f_host, 1e3*G.t_f_dev, 1e3*G.t_final, G.tier_steps,
                     1e3*G.t_tiers, 1e3*G.t_dilate);
         if(G.A == 0){ res->S[0] = res->S[1] = res->S[2] = 0; res->t = now() - t0; return SFX_OK; }
         { double r = G.e2 > 0 ? G.e1/G.e2 : 0; res->trunc = (r > 0 && r < 1) ? G.e1*r/(1 - r) : G.e1; }
