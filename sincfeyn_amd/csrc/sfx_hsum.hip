@@ -34,11 +34,11 @@ struct JobBufs {
     DevBuf lsk, lfu, lcnt;                  // a refinement block's cells split by |.|-sum class; counts
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
     DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
-           dkey, dns, dtr, skey, skey2, sortt;
+           dkey, skey, skey2, sortt;
     template<class F> void each(F f){
         DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc, &vfe,
                          &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
-                         &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &dns, &dtr, &skey, &skey2, &sortt};
+                         &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &skey, &skey2, &sortt};
         for(DevBuf *b : all) f(*b);
     }
     void release(){ each([](DevBuf &b){ b.release(); }); }
