@@ -69,25 +69,45 @@ def write_registry(names):
     return path
 
 
-def unit_flags(src):
-    """extra hipcc flags of one translation unit: the integrands emitted with node_split (their FP64 point
+TUNE_TABLE = os.path.join(PKG, 'tune_table.json')
+# the build variants the tuner (tune.py) measures; a graph's entry in tune_table.json names one
+VARIANTS = {
+    'base': [],
+    'w3': ['-DSFX_LIST_WAVES0=3', '-DSFX_LIST_WAVES1=3'],
+    'w4': ['-DSFX_LIST_WAVES0=4', '-DSFX_LIST_WAVES1=3'],
+    'burr': ['-mllvm', '-pre-RA-sched=list-burr'],
+    'burr_w3': ['-mllvm', '-pre-RA-sched=list-burr', '-DSFX_LIST_WAVES0=3', '-DSFX_LIST_WAVES1=3'],
+    'burr_w4': ['-mllvm', '-pre-RA-sched=list-burr', '-DSFX_LIST_WAVES0=4', '-DSFX_LIST_WAVES1=3'],
+}
+
+
+def unit_flags(src, use_table=True):
+    """extra hipcc flags of one translation unit.  The integrands emitted with node_split (their FP64 point
     function as one switch case per forest node) are scheduled bottom-up for register need, which keeps
     one case within the 256 VGPRs of two waves per SIMD (g3_lbl: 244; the default scheduler's 258 leaves
-    one wave).  Scheduling only: the same instructions on the same operands."""
+    one wave).  Other graphs take the variant that tune_table.json names for them (measured by tune.py:
+    occupancy of the refinement kernels, scheduler).  Scheduling and register allocation only: the same
+    instructions on the same operands, so every variant gives the same values bit for bit."""
     meta = src[:-4] + '.json'
     if src.endswith('.hip') and os.path.exists(meta):
         with open(meta) as f:
             if json.load(f).get('node_split'):
                 return ['-mllvm', '-pre-RA-sched=list-burr']
+    name = os.path.basename(src)
+    if use_table and name.startswith('k_') and os.path.exists(TUNE_TABLE):
+        with open(TUNE_TABLE) as f:
+            v = json.load(f).get('graphs', {}).get(name[2:-4])
+        if v:
+            return list(VARIANTS[v])
     return []
 
 
-def compile_one(src, obj, deps, force):
+def compile_one(src, obj, deps, force, flags):
     if not force and os.path.exists(obj):
         t = os.path.getmtime(obj)
         if all(os.path.getmtime(d) <= t for d in [src] + deps):
             return obj, 0.0, ''
-    cmd = [HIPCC] + HIPFLAGS + unit_flags(src) + ['-c', src, '-o', obj + '.tmp']
+    cmd = [HIPCC] + HIPFLAGS + flags + ['-c', src, '-o', obj + '.tmp']
     import time
     t0 = time.time()
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -97,10 +117,15 @@ def compile_one(src, obj, deps, force):
     return obj, time.time() - t0, p.stderr
 
 
-def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None):
+def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None, libdir=None, objdir=None, variant=None):
+    """variant (a VARIANTS key): every integrand unit with those flags instead of the tuned table, into
+    libdir/objdir (the tuner's trial builds)"""
+    libdir = libdir or LIBDIR
+    objdir = objdir or OBJ
+    lib = os.path.join(libdir, 'libsfx.so')
     os.makedirs(GEN, exist_ok=True)
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(libdir, exist_ok=True)
     if os.path.isdir(KERNELS):
         names = emit_all(classes, only)
         log('emitted %d integrands from %s' % (len(names), KERNELS))
@@ -118,33 +143,41 @@ def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None):
     hdrs = [os.path.join(CSRC, h) for h in ('sfx_integrand.hpp', 'sfx_dd.hpp', 'sfx_qd.hpp', 'sfx_abi_types.h')]
     rt_hdrs = hdrs + [os.path.join(CSRC, h) for h in ('sfx_device.hpp', 'sfx_engine.hpp', 'sfx_grow.inc', 'sfx_scan.inc', 'sfx_ckpt.inc')]
     rt_hdrs.append(os.path.join(ROOT, 'include', 'sfx.h'))
-    units = [(os.path.join(GEN, 'k_%s.hip' % n), os.path.join(OBJ, 'k_%s.o' % n), hdrs + [os.path.join(GEN, 'k_%s.inc' % n)])
+    tdeps = [TUNE_TABLE] if os.path.exists(TUNE_TABLE) and variant is None else []
+
+    def kflags(src):
+        if variant is None:
+            return unit_flags(src)
+        return unit_flags(src, use_table=False) or list(VARIANTS[variant])
+    units = [(os.path.join(GEN, 'k_%s.hip' % n), os.path.join(objdir, 'k_%s.o' % n), hdrs + [os.path.join(GEN, 'k_%s.inc' % n)] + tdeps)
              for n in names]
-    units.append((reg, os.path.join(OBJ, 'sfx_registry.o'), hdrs))
+    units.append((reg, os.path.join(objdir, 'sfx_registry.o'), hdrs))
     for rt in ('sfx_runtime.hip', 'sfx_hsum.hip'):
-        units.append((os.path.join(CSRC, rt), os.path.join(OBJ, rt.replace('.hip', '.o')), rt_hdrs))
+        units.append((os.path.join(CSRC, rt), os.path.join(objdir, rt.replace('.hip', '.o')), rt_hdrs))
     # largest sources first, so the long compiles start early
     units.sort(key=lambda u: -os.path.getsize(u[0]))
     jobs = jobs or min(16, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = [ex.submit(compile_one, s, o, d, force) for s, o, d in units]
+        futs = [ex.submit(compile_one, s, o, d, force, kflags(s)) for s, o, d in units]
         for f in cf.as_completed(futs):
             o, dt, _ = f.result()
             objs.append(o)
             if dt > 20:
                 log('%s: %.0f s' % (os.path.basename(o), dt))
     objs.sort()
-    tmp = LIB + '.tmp'
+    tmp = lib + '.tmp'
     cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError('link failed:\n%s' % p.stderr[-4000:])
-    os.replace(tmp, LIB)
-    log('linked %s (%d graphs)' % (os.path.relpath(LIB, ROOT), len(names)))
+    os.replace(tmp, lib)
+    log('linked %s (%d graphs)' % (os.path.relpath(lib, ROOT), len(names)))
+    if variant is not None:
+        return lib
     build_cli()
     opcounts(names, jobs, force)
-    return LIB
+    return lib
 
 
 def _opcount_one(name, force):

@@ -87,11 +87,12 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
 // code and the register pressure drops), 1 every point computes them, 2 chosen per wave.
 constexpr int MODE_NODES = 0, MODE_LIST = 1;
 
+// the body of the evaluation kernels (k_eval for node lists, k_eval_list for a block's cells)
 template<class G, int MODE, int AB>
-__global__ void __launch_bounds__(SFX_BLOCK)
-k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
-       long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout,
-       uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+__device__ __forceinline__ void eval_points(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list,
+                                            const uint32_t *__restrict__ cnt, long n, int st, const sfx_eval_params_t &P,
+                                            double *__restrict__ vout, uint8_t *__restrict__ flout,
+                                            uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
     constexpr int D = G::D;
     long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
     if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
@@ -157,6 +158,32 @@ k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, co
         base = __shfl(base, 0, 64);
         if(q) ddidx[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = (uint32_t)o;
     }
+}
+
+// Occupancy of the refinement kernels, chosen per graph and class by measurement (sincfeyn_amd/tune.py
+// writes the table that build.py turns into -DSFX_LIST_WAVES0=n / -DSFX_LIST_WAVES1=n for the graph's
+// translation unit): at least n waves per SIMD, i.e. at most 512/n VGPRs; 1 leaves the compiler's choice.
+#ifndef SFX_LIST_WAVES0
+#define SFX_LIST_WAVES0 1
+#endif
+#ifndef SFX_LIST_WAVES1
+#define SFX_LIST_WAVES1 1
+#endif
+#define SFX_LIST_ATTR __attribute__((amdgpu_waves_per_eu(AB == 0 ? SFX_LIST_WAVES0 : SFX_LIST_WAVES1)))
+
+template<class G, int MODE, int AB>
+__global__ void __launch_bounds__(SFX_BLOCK)
+k_eval(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
+       long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout,
+       uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+    eval_points<G, MODE, AB>(pts, list, cnt, n, st, P, vout, flout, ddidx, ddcnt);
+}
+template<class G, int AB>
+__global__ void __launch_bounds__(SFX_BLOCK) SFX_LIST_ATTR
+k_eval_list(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
+            long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout,
+            uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+    eval_points<G, MODE_LIST, AB>(pts, list, cnt, n, st, P, vout, flout, ddidx, ddcnt);
 }
 
 // the high-precision recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
@@ -284,8 +311,8 @@ struct sfx_graph_entry {
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SFX_GRAPH_ENTRY(NAME, G, VOLKOV, L, D, W)                                                            \
 template __global__ void sfx::k_eval<G, sfx::MODE_NODES, 2>(SFX_EVAL_ARGS);                                  \
-template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 0>(SFX_EVAL_ARGS);                                   \
-template __global__ void sfx::k_eval<G, sfx::MODE_LIST, 1>(SFX_EVAL_ARGS);                                   \
+template __global__ void sfx::k_eval_list<G, 0>(SFX_EVAL_ARGS);                                              \
+template __global__ void sfx::k_eval_list<G, 1>(SFX_EVAL_ARGS);                                              \
 template __global__ void sfx::k_dd<G, false>(SFX_DD_ARGS);                                                   \
 template __global__ void sfx::k_dd<G, true>(SFX_DD_ARGS);                                                    \
 template __global__ void sfx::k_qd<G, false>(SFX_DD_ARGS);                                                   \
@@ -302,8 +329,8 @@ static hipError_t sfx_eval_##NAME(hipStream_t s, int mode, int ab, const sfx_cel
     if(mode == sfx::MODE_NODES)                                                                              \
         hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_NODES, 2>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
     else if(ab == 0)                                                                                         \
-        hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 0>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
-    else hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_LIST, 1>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
+        hipLaunchKernelGGL((sfx::k_eval_list<G, 0>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt);  \
+    else hipLaunchKernelGGL((sfx::k_eval_list<G, 1>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt);     \
     return hipGetLastError();                                                                                \
 }                                                                                                            \
 static hipError_t sfx_dd_##NAME(hipStream_t s, int cells, const sfx_cell_t *pts, int st, const uint32_t *idx, \
@@ -342,7 +369,7 @@ static hipError_t sfx_pt_##NAME(hipStream_t s, long n, const double *xhi, const 
 extern "C" const sfx_graph_entry SFX_E_##NAME = {#NAME, VOLKOV, L, D, W,                                      \
     sfx_eval_##NAME, sfx_dd_##NAME, sfx_raw_##NAME, sfx_pt_##NAME, sfx_qd_##NAME, G::NL, G::FIXED,           \
     G::HAS_PT ? 1 : 0, G::QD ? 1 : 0,                                                                        \
-    {(const void*)&sfx::k_eval<G, sfx::MODE_NODES, 2>, (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 0>,     \
-     (const void*)&sfx::k_eval<G, sfx::MODE_LIST, 1>,                                                        \
+    {(const void*)&sfx::k_eval<G, sfx::MODE_NODES, 2>, (const void*)&sfx::k_eval_list<G, 0>,                 \
+     (const void*)&sfx::k_eval_list<G, 1>,                                                                   \
      (const void*)&sfx::k_dd<G, false>, (const void*)&sfx::k_dd<G, true>}};
 #endif

@@ -295,8 +295,12 @@ def emit(src_path, out_dir):
     meta = dict(desc, source=os.path.basename(src_path), functions=len(names),
                 chunk_functions=sum(1 for n in names if _is_chunk(n)), static_ops=count_ops(body),
                 node_split=ncases)
-    with open(os.path.join(out_dir, 'k_%s.json' % name), 'w') as f:
-        json.dump(meta, f, indent=1)
+    mp = os.path.join(out_dir, 'k_%s.json' % name)
+    text = json.dumps(meta, indent=1)
+    if not (os.path.exists(mp) and open(mp).read() == text):
+        with open(mp + '.tmp', 'w') as f:     # replaced whole: a build running beside this one reads it
+            f.write(text)
+        os.replace(mp + '.tmp', mp)
     return out, meta
 
 

@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, '_lib', 'libsfx.so')
+# SFX_LIB: another build of the library (the tuner's trial builds of sincfeyn_amd/tune.py)
+LIB_PATH = os.environ.get('SFX_LIB') or os.path.join(PKG, '_lib', 'libsfx.so')
 
 
 class SfxError(RuntimeError):
