@@ -198,6 +198,22 @@ int sfx_eval_raw(int graph, long n, const double *sf, int prec, double *out);
 int sfx_eval_finished(int graph, double h, int K, long n, const int16_t *k, const uint8_t *skip,
                       double abserr, const sfx_opts_t *opts, double *v, uint8_t *flags, long *nq);
 
+/* sfx_eval_finished for callers that cannot allocate: no counterpart in the reference; for code under
+ * HIP graph capture, tight serving loops and harnesses that forbid hipMalloc.  Everything is
+ * device-resident and the call only enqueues kernels on `stream` (NULL: the library's stream) and
+ * returns: dEX[k+K] = e^{kh} as the caller computed it (2K+1 doubles), dk the n points' coordinates
+ * (int16, D per point, |k| <= K), dskip the skip bytes (or NULL); out dv (values), dflags (SFX_PT_* bits;
+ * SFX_PT_BADK for a point with a coordinate outside the box, its value 0) and *dnq (uint32, the fallback
+ * count).  Its working memory is a fixed device array inside the library (about 7.9 MB, held until the
+ * process ends): n <= SFX_NOALLOC_MAX_POINTS, larger n is refused with SFX_EINVAL.  Every call on a GPU
+ * uses that one array, so calls that overlap on one device (other streams or threads) race; one call after
+ * another on a stream is safe.  Same values, flags and count as sfx_eval_finished. */
+#define SFX_NOALLOC_MAX_POINTS (1L << 18)
+#define SFX_PT_BADK 128
+int sfx_eval_finished_noalloc(int graph, int K, double h, const double *dEX, long n, const int16_t *dk,
+                              const uint8_t *dskip, double abserr, const sfx_opts_t *opts, double *dv,
+                              uint8_t *dflags, uint32_t *dnq, void *stream);
+
 /* ---- the hierarchical sum ----
  * hsum(g, h, N, eps, top) for one graph on the device: K = (N-1)/2 rounded up to a multiple of
  * 2^(top+1) (lattice.c:16; K <= 127), 0 <= top <= 12.  The coarsest level (the support-following grow, or

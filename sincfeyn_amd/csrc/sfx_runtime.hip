@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <algorithm>
 #include <vector>
 #include "sfx_device.hpp"
 
@@ -339,6 +340,42 @@ hipError_t scan_keep(hipStream_t s, const ScanArgs &a){
 using namespace sfx;
 
 // ---------------------------------------------------------------------------------------------------
+// The entry point that allocates nothing (sfx_eval_finished_noalloc): its working memory is this
+// file-scope device array.  In host code `scratch` only names the symbol; the memory is reached through
+// the device address hipGetSymbolAddress returns, taken once in sfx_init with the device current.
+// Layout: point records | round terms | fallback list | flag bytes | bad-coordinate marks | counter.
+static constexpr long NA_MAX = SFX_NOALLOC_MAX_POINTS;
+static constexpr size_t NA_CELLS = 0, NA_E2 = NA_CELLS + (size_t)NA_MAX*sizeof(sfx_cell_t), NA_IDX = NA_E2 + (size_t)NA_MAX*8,
+                        NA_FL = NA_IDX + (size_t)NA_MAX*4, NA_BAD = NA_FL + (size_t)NA_MAX, NA_CNT = NA_BAD + (size_t)NA_MAX,
+                        NA_BYTES = NA_CNT + 256;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+static unsigned char *g_scratch = nullptr;          // scratch's device address on the library's device
+
+// device int16 coordinates (D per point) and skip bytes -> point records; thread 0 also clears the
+// fallback counter (a kernel, not a memset, so the sequence can be captured into a HIP graph); a point with
+// a coordinate outside |k| <= K is evaluated at k = 0 and marked SFX_PT_BADK by k_na_finish
+__global__ void k_na_pack(const int16_t *k, const uint8_t *skip, long n, int D, int K, sfx_cell_t *cells, uint8_t *bad, uint32_t *cnt){
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    if(i == 0) *cnt = 0;
+    if(i >= n) return;
+    sfx_cell_t c; for(int d = 0; d < SFX_MAXD; d++) c.k[d] = 0; c.pad = 0;
+    bool ok = true;
+    for(int d = 0; d < D; d++){ int x = k[i*D + d]; ok = ok && x >= -K && x <= K; c.k[d] = (int8_t)x; }
+    if(!ok) for(int d = 0; d < SFX_MAXD; d++) c.k[d] = 0;
+    c.fl = skip && skip[i] ? 1 : 0;
+    cells[i] = c; bad[i] = ok ? 0 : 1;
+}
+// the evaluation's flag bytes -> the SFX_PT_* bits of sfx_eval_finished; the fallback count to *nq
+__global__ void k_na_finish(const uint8_t *fl, const uint8_t *bad, long n, double *v, uint8_t *flags, const uint32_t *cnt, uint32_t *nq){
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    if(i == 0) *nq = *cnt;
+    if(i >= n) return;
+    const unsigned f = fl[i];
+    if(bad[i]){ v[i] = 0; flags[i] = SFX_PT_BADK; return; }
+    flags[i] = (uint8_t)((f & (SFX_FL_NEAR | SFX_FL_DD)) | ((f & SFX_FL_UNRES) ? SFX_PT_UNRES : 0));
+}
+
+// ---------------------------------------------------------------------------------------------------
 // C ABI: library and registry
 extern "C" const char *sfx_last_error(void){ return g_err; }
 
@@ -360,6 +397,11 @@ extern "C" int sfx_init(int device){
         hipMemPool_t pool; uint64_t keep = (uint64_t)16 << 30;
         SFX_HIP(hipDeviceGetDefaultMemPool(&pool, device));
         SFX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
+    {
+        void *p = nullptr;
+        SFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch = (unsigned char*)p;
     }
     L.device = device; L.init = true;
     char b[512];
@@ -432,6 +474,42 @@ extern "C" int sfx_graph_selected(int i, const char *s){
 // per-point primitives
 static int check_graph(int graph){
     if(graph < 0 || graph >= SFX_NGRAPHS){ set_error("graph index %d out of range (%d graphs)", graph, SFX_NGRAPHS); return SFX_EINVAL; }
+    return SFX_OK;
+}
+
+extern "C" int sfx_eval_finished_noalloc(int graph, int K, double h, const double *dEX, long n, const int16_t *dk,
+                                         const uint8_t *dskip, double abserr, const sfx_opts_t *opts, double *dv,
+                                         uint8_t *dflags, uint32_t *dnq, void *stream){
+    int rc;
+    if((rc = require_init()) || (rc = check_graph(graph))) return rc;
+    if(n < 0 || n > NA_MAX){
+        set_error("sfx_eval_finished_noalloc: %ld points; this entry point takes at most %ld (SFX_NOALLOC_MAX_POINTS)", n, NA_MAX);
+        return SFX_EINVAL;
+    }
+    if(K < 1 || K > 127 || !(h > 0) || !dEX || !dnq || (n && (!dk || !dv || !dflags))){
+        set_error("sfx_eval_finished_noalloc: bad arguments"); return SFX_EINVAL;
+    }
+    if(!g_scratch){ set_error("sfx_eval_finished_noalloc: no scratch address (sfx_init)"); return SFX_EDEVICE; }
+    const sfx_graph_entry *g = SFX_GRAPHS[graph];
+    sfx_opts_t o; if(opts) o = *opts; else sfx_opts_default(&o);
+    if(o.qd_fallback && !g->has_qd){ set_error("%s: no quad-double kernel (two-loop graphs only)", g->name); return SFX_EUNSUP; }
+    sfx_eval_params_t P{};
+    P.EX = dEX; P.K = K; P.H = h; P.abskip = o.abskip; P.abserr = abserr; P.nearf = o.nearf; P.ddslack = o.ddslack;
+    P.dropunres = o.dropunres; P.abcheck = o.abcheck; P.raw_ab = nullptr;
+    hipStream_t s = stream ? (hipStream_t)stream : lib().stream;
+    sfx_cell_t *cells = (sfx_cell_t*)(g_scratch + NA_CELLS);
+    double *e2 = (double*)(g_scratch + NA_E2);
+    uint32_t *idx = (uint32_t*)(g_scratch + NA_IDX), *cnt = (uint32_t*)(g_scratch + NA_CNT);
+    uint8_t *fl = g_scratch + NA_FL, *bad = g_scratch + NA_BAD;
+    const unsigned nb = (unsigned)std::max<long>((n + 255)/256, 1);
+    hipLaunchKernelGGL(k_na_pack, dim3(nb), dim3(256), 0, s, dk, dskip, n, g->D, K, cells, bad, cnt);
+    SFX_HIP(hipGetLastError());
+    if(n){
+        SFX_HIP(g->eval(s, MODE_NODES, 2, cells, nullptr, nullptr, n, 0, &P, dv, fl, idx, cnt));
+        SFX_HIP((o.qd_fallback ? g->qd : g->dd)(s, 0, cells, 0, idx, cnt, n, &P, dv, e2, fl));
+    }
+    hipLaunchKernelGGL(k_na_finish, dim3(nb), dim3(256), 0, s, fl, bad, n, dv, dflags, cnt, dnq);
+    SFX_HIP(hipGetLastError());
     return SFX_OK;
 }
 
