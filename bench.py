@@ -44,7 +44,7 @@ SLICES = 10
 PEAK_FP64 = 78.6          # TFLOP/s, MI355X FP64 vector (FMA counted as 2)
 KIND = {0: 'level_eval', 1: 'level_dd', 2: 'level_sum', 3: 'nodes_eval', 4: 'nodes_dd', 5: 'level_eval_skip',
         6: 'level_split'}
-KNAME = {'level_eval': 'k_eval<%s, list, AB=1>', 'level_eval_skip': 'k_eval<%s, list, AB=0>'}
+KNAME = {'level_eval': 'k_eval_list<%s, AB=1>', 'level_eval_skip': 'k_eval_list<%s, AB=0>'}
 # bounded samples of the workload: the plain GPU path (the reference's host driver is bound by its serial
 # sums, ~2.5e8 points/s on the largest jobs) and the reference CPU build.  Both hold a D = 7 integral
 # (D = 7 jobs are 91% of the workload's points); the plain sample also the light-by-light integral, whose
@@ -175,7 +175,7 @@ def roofline(S, opc):
     kname = KNAME[top['kind']] % top['graph'] if top['kind'] in KNAME else top['kind']
     rl = {'bound': 'fp64-valu', 'kernel': kname, 'achieved': round(ach, 3), 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
           'frac': round(ach/PEAK_FP64, 4), 'traffic': None, 'rate': 'algorithmic: the source operations of the '
-          'integrand (+ - * /, SURVEY.md Appendix A counting) x valid points per launch / mean launch time',
+          'integrand (each + - * / counted once) x valid points per launch / mean launch time',
           'launches': top['launches'], 'avg_launch_ms': round(top['ms']/top['launches'], 4),
           'points_per_launch': top['points']/top['launches'], 'flops_per_launch': fl/top['launches'],
           'share_of_device_time': round(top['ms']/tot_ms, 3)}
@@ -191,8 +191,14 @@ def roofline(S, opc):
         t = json.load(open(p))
         if t.get('kernel') == kname:
             ex = t['executed_fp64_flops_per_dispatch']
+            c = t['per_dispatch']
+            n64 = sum(c['SQ_INSTS_VALU_%s_F64' % k] for k in ('ADD', 'MUL', 'FMA', 'TRANS'))
+            # with FP contraction off nearly every FP64 instruction is an ADD or a MUL: one operation per
+            # lane per issue, so the issue limit of this instruction mix is half the FMA peak, times 1 + FMA share
+            lim = PEAK_FP64/2*(1 + c['SQ_INSTS_VALU_FMA_F64']/n64)
             rl['executed'] = {'achieved': round(ex/avg_s/1e12, 3), 'frac': round(ex/avg_s/1e12/PEAK_FP64, 4),
                               'flops_per_launch': ex, 'fp64_issue_busy': t.get('fp64_issue_busy'),
+                              'issue_limit_of_mix': round(lim, 2), 'frac_of_mix_limit': round(ex/avg_s/1e12/lim, 4),
                               'source': os.path.relpath(p, ROOT)}
     agg = [e for e in ent if e['kind'] in ('level_eval', 'level_eval_skip')]
     if agg:

@@ -1,0 +1,72 @@
+"""profsum.py -- reduce rocprofv3 CSV output to the small per-kernel tables kept under profiles/.
+
+    python -m sincfeyn_amd.profsum counters OUT.json CSV... [--match REGEX]
+        mean, per dispatch, of every counter in the counter-collection CSVs (rocprofv3 --pmc ...
+        --output-format csv), per kernel name matching REGEX, with the dispatch count and grid size
+    python -m sincfeyn_amd.profsum stats OUT.md KERNEL_STATS_CSV [--top N]
+        the first N rows of a kernel-stats CSV (rocprofv3 --kernel-trace --stats) as a markdown table
+
+The committed tables are what bench.py reads: the executed FP64 operations per dispatch of its roofline
+kernel (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64) and its HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE),
+each scaled by what the counter reads for the library's copy kernel of known size.
+"""
+import argparse
+import collections
+import csv
+import json
+import re
+import sys
+
+
+def counters(out, files, match):
+    rx = re.compile(match)
+    acc = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    grid = collections.defaultdict(float)
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                name = row.get('Kernel_Name', '')
+                if not rx.search(name):
+                    continue
+                did = row.get('Dispatch_Id') or row.get('Correlation_Id')
+                acc[name][row['Counter_Name']] += float(row['Counter_Value'])
+                if did not in disp[name]:
+                    disp[name].add(did)
+                    grid[name] += float(row.get('Grid_Size', 0) or 0)
+    res = {}
+    for name, c in acc.items():
+        n = len(disp[name])
+        res[name] = {'dispatches': n, 'mean_grid_size': grid[name]/n if n else 0,
+                     'per_dispatch': {k: v/n for k, v in sorted(c.items())}}
+    with open(out, 'w') as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print('%d kernels -> %s' % (len(res), out))
+
+
+def stats(out, fn, top):
+    with open(fn) as f:
+        rows = list(csv.DictReader(f))
+    keys = [k for k in ('Name', 'Calls', 'TotalDurationNs', 'AverageNs', 'Percentage', 'MinNs', 'MaxNs') if rows and k in rows[0]]
+    lines = ['| ' + ' | '.join(keys) + ' |', '|' + '---|'*len(keys)]
+    for r in rows[:top]:
+        lines.append('| ' + ' | '.join(r[k].replace('|', '/') for k in keys) + ' |')
+    with open(out, 'w') as f:
+        f.write('\n'.join(lines) + '\n')
+    print('%d rows -> %s' % (min(top, len(rows)), out))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest='cmd', required=True)
+    c = sub.add_parser('counters'); c.add_argument('out'); c.add_argument('files', nargs='+'); c.add_argument('--match', default='.')
+    s = sub.add_parser('stats'); s.add_argument('out'); s.add_argument('file'); s.add_argument('--top', type=int, default=40)
+    a = ap.parse_args(argv)
+    if a.cmd == 'counters':
+        counters(a.out, a.files, a.match)
+    else:
+        stats(a.out, a.file, a.top)
+
+
+if __name__ == '__main__':
+    sys.exit(main())
