@@ -44,6 +44,7 @@ SLICES = 10
 PEAK_FP64 = 78.6          # TFLOP/s, MI355X FP64 vector (FMA counted as 2)
 KIND = {0: 'level_eval', 1: 'level_dd', 2: 'level_sum', 3: 'nodes_eval', 4: 'nodes_dd', 5: 'level_eval_skip',
         6: 'level_split'}
+JOB_KW = {}               # run options of every job (sfx_opts_t fields), set from the command line
 KNAME = {'level_eval': 'k_eval_list<%s, AB=1>', 'level_eval_skip': 'k_eval_list<%s, AB=0>'}
 # bounded samples of the workload: the plain GPU path (the reference's host driver is bound by its serial
 # sums, ~2.5e8 points/s on the largest jobs) and the reference CPU build.  Both hold a D = 7 integral
@@ -77,7 +78,7 @@ def run_batched(S, steps, lat, out):
     """the batched path: every job of the given steps submitted to the engine in step order (widest first
     within a step), then waited for; a job's latency is submit -> result"""
     ev = 0
-    ids = [(S.submit(gi, h, N, EPS, top), (name, h)) for step in steps for gi, name, D, h, N, top in step]
+    ids = [(S.submit(gi, h, N, EPS, top, **JOB_KW), (name, h)) for step in steps for gi, name, D, h, N, top in step]
     for j, key in ids:
         r, tq, tr = S.wait(j)
         lat.append(tq + tr)
@@ -91,7 +92,7 @@ def run_per_graph(S, jobs, lat, out):
     ev = 0
     for gi, name, D, h, N, top in jobs:
         t0 = time.perf_counter()
-        r = S.hsum(gi, h, N, EPS, top)
+        r = S.hsum(gi, h, N, EPS, top, **JOB_KW)
         lat.append(time.perf_counter() - t0)
         ev += r['nev']
         out[(name, h)] = r
@@ -120,16 +121,12 @@ def run_reference_sample(sample, gpu, cores):
 
 
 def fixtures():
-    """the reference's committed results without FP contraction: {(graph, h): record} at eps 1e-12"""
-    out = {}
-    for p in glob.glob(os.path.join(ROOT, 'tests', 'golden', 'fixtures', 'nc_h*_eps1e-12.txt')):
-        h = float(os.path.basename(p).split('_')[1][1:])
-        for line in open(p):
-            if line.startswith('#R '):
-                t = line.split()
-                d = {t[i]: t[i + 1] for i in range(2, len(t) - 1, 2)}
-                out[(t[1], h)] = d
-    return out
+    """the reference's committed results without FP contraction: {(graph, h): record} at eps 1e-12, the
+    counts taken from the reference driver's GPU build where one is committed (refgpu_*: g3_lbl, whose CPU
+    build evaluates a differently rounded double half of the kernel)"""
+    from sincfeyn_amd import serve
+    return {(g, h): r for (g, h, N, eps), r in serve.reference_records(
+        os.path.join(ROOT, 'tests', 'golden', 'fixtures')).items() if eps == EPS}
 
 
 def check_reference(res, fx):
@@ -224,7 +221,10 @@ def main():
     ap.add_argument('--profile-out', default=None, help='write the per-kernel table of the profiled pass as JSON here')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
+    ap.add_argument('--graph-capture', type=int, default=0,
+                    help='1: every refinement level\'s block chain runs as one captured HIP graph (sfx_opts_t.graph_capture)')
     a = ap.parse_args()
+    JOB_KW['graph_capture'] = a.graph_capture
     if a.gpus != 1:
         print('bench.py: this build measures one GPU; the Sinc sums are not split across GPUs here '
               '(--gpus %d is not supported)' % a.gpus, file=sys.stderr)
@@ -324,7 +324,8 @@ def main():
                                '(N 97/121/139, top 1/1/2), eps 1e-12: 138 jobs through the job engine; a step is one '
                                'of ten fixed slices of the jobs (widest first, dealt round-robin)',
                    'jobs': len(jobs), 'jobs_per_step': len(slices[0]), 'eps': EPS, 'points_per_step': ev//a.steps,
-                   'path': 'batched', 'workers': a.workers, 'precision': 'FP64 bulk, double-double where flagged'},
+                   'path': 'batched', 'workers': a.workers, 'precision': 'FP64 bulk, double-double where flagged',
+                   'graph_capture': a.graph_capture},
         'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat),
                       'note': 'submit -> result with every timed step queued at once'},
         'per_graph': {'value': ev_pg/dt_pg, 'wall_s': dt_pg, 'jobs': len(jobs),

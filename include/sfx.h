@@ -89,6 +89,11 @@ typedef struct {
                              full anyway, and those that needed the fallback are counted (abmiss, eval.c:11-12) */
     int growskip;         /* SINCQED_GROWSKIP (default 2): at top = 0, a frontier point of the grow skips the |.|
                              sums for up to this many generations after a checked point (sumkey.h:165-168) */
+    int graph_capture;    /* 1: the chain of launches of each refinement level (every block's class split,
+                             evaluations, fallback and reductions) is captured on the job's stream as one HIP
+                             graph, instantiated and launched once; ignored while checkpoints or the per-kernel
+                             timers are on (both need the host between blocks).  Same results; 0 (default)
+                             launches the kernels directly. */
 } sfx_opts_t;
 
 typedef struct {

@@ -300,6 +300,9 @@ __global__ void __launch_bounds__(256) k_cell_split(const sfx_cell_t *cells, lon
 // before it (a resumed checkpoint), or at 0 with nothing
 struct LevelStart { long c0 = 0; const LevelAcc *acc = nullptr; const std::vector<sfx_cell_t> *nxt = nullptr; };
 
+static int level_blocks(Job &J, int lev, long ncur, const LevelStart &ls, CkRun &ck, long Bc, double pw, double thr,
+                        long &prof_nv, long &prof_nsk, long &prof_nq);
+
 // one refinement level on the device: the children of cur[c0..ncur) in blocks; returns the level's sums
 // (a checkpoint between blocks when one is due: ck)
 static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, LevelAcc *out, const LevelStart &ls, CkRun &ck){
@@ -328,6 +331,39 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
     if(nblk) hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)std::min<long>((nblk*4 + 255)/256, 1024)), dim3(256), 0, s, B.lcnt.as<uint32_t>(), nblk*4);
     long prof_nv = 0, prof_nsk = 0, prof_nq = 0;
     if(ls.acc){ prof_nv = (long)ls.acc->nv; prof_nsk = (long)ls.acc->nskip; prof_nq = (long)ls.acc->nq; }
+    // with graph_capture the blocks' launches are recorded on the job's stream (thread-local capture: the
+    // engine's other workers keep allocating on their own streams) and run as one graph; the chain holds
+    // kernels only (no memset or copy nodes), and nothing below reads the device back between blocks
+    // unless the timers or checkpoints are on, which turn the capture off
+    const bool cap = J.o.graph_capture && !prof().on && !ck.ck->on && nblk > 0;
+    if(cap) SFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = level_blocks(J, lev, ncur, ls, ck, Bc, pw, thr, prof_nv, prof_nsk, prof_nq);
+    if(cap){
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &gr);
+        if(rc == SFX_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        if(rc == SFX_OK){
+            hipGraphExec_t ge = nullptr;
+            e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+            if(e == hipSuccess) e = hipGraphLaunch(ge, s);
+            // the executable graph may be destroyed once its launch is queued (the work completes)
+            if(ge) (void)hipGraphExecDestroy(ge);
+            if(e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate/Launch", __FILE__, __LINE__);
+        }
+        if(gr) (void)hipGraphDestroy(gr);
+    }
+    if(rc) return rc;
+    { RB rb = {out, B.acc.p, sizeof *out}; if((rc = read_back(X, s, &rb, 1))) return rc; }
+    *nnext = (long)out->nkept;
+    return SFX_OK;
+}
+
+// the blocks of one refinement level from cell ls.c0 on (the launches only, unless the timers or a due
+// checkpoint read the device back)
+static int level_blocks(Job &J, int lev, long ncur, const LevelStart &ls, CkRun &ck, long Bc, double pw, double thr,
+                        long &prof_nv, long &prof_nsk, long &prof_nq){
+    JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; hipStream_t s = J.s; int rc;
+    const int D = J.D, st = 1 << lev; const bool last = lev == 0;
     for(long c0 = ls.c0; c0 < ncur; c0 += Bc){
         long nc = ncur - c0 < Bc ? ncur - c0 : Bc, n = nc << D;
         const sfx_cell_t *cells = B.cur.as<sfx_cell_t>() + c0;
@@ -391,8 +427,6 @@ static int refine_level(Job &J, int lev, long ncur, long *nnext, double ref, Lev
             }
         }
     }
-    { RB rb = {out, B.acc.p, sizeof *out}; if((rc = read_back(X, s, &rb, 1))) return rc; }
-    *nnext = (long)out->nkept;
     return SFX_OK;
 }
 

@@ -11,6 +11,7 @@ set; h uniform over `hs`, eps uniform over `epss`; N = 2 ceil(48/h) + 1 (the rep
 top from the reference's rule (main.c:211-215).
 
     python -m sincfeyn_amd.serve --rate 10 --jobs 100 --seed 1 [--graphs all2,all3] [--workers 16]
+    python -m sincfeyn_amd.serve --sweep 1,2,5,10,20,50 --jobs 150 --seed 7 --check tests/golden/fixtures
 """
 import argparse
 import json
@@ -114,6 +115,72 @@ def serve(stream, results=None, progress=False, wait_timeout=600.0):
             'offered_rate': len(stream)/stream[-1][0] if stream else 0}
 
 
+def reference_records(folder):
+    """{(graph, h, N, eps): {field: text}} from the "#R" result records in folder/nc_h*_N*_eps*.txt: one line
+    per graph with every field of the reference's res_t (the reference built without FP contraction, with a
+    one-line results printer; tests/golden/make_fixtures.py).  Where folder/refgpu_h*_N*_eps*.txt holds a
+    record of the same job (the reference driver's GPU build), its counts replace the CPU build's: they
+    differ for a kernel whose CPU double half rounds differently from the QREAL half that the GPU evaluates"""
+    import glob
+    import os
+    out = {}
+    for build in ('nc', 'refgpu'):
+        for p in glob.glob(os.path.join(folder, build + '_h*_N*_eps*.txt')):
+            _, h, N, eps = os.path.basename(p)[:-4].split('_')
+            for line in open(p):
+                if line.startswith('#R '):
+                    t = line.split()
+                    key = (t[1], float(h[1:]), int(N[1:]), float(eps[3:]))
+                    rec = {t[i]: t[i + 1] for i in range(2, len(t) - 1, 2)}
+                    if build == 'nc':
+                        out[key] = rec
+                    elif key in out:
+                        out[key].update({k: rec[k] for k in ('nev', 'nq', 'kept', 'skipped')})
+    return out
+
+
+def check_job(r, rec, rel=1e-12):
+    """a job's result against the reference's record: evaluation, fallback and kept counts equal, S_h within
+    rel (the reference built without FP contraction, whose arithmetic the device reproduces)"""
+    S0 = float(rec['S0'])
+    d = abs(r['S'][0] - S0)/abs(S0) if S0 else abs(r['S'][0])
+    same = (r['nev'], r['nq'], r['kept_top']) == (int(rec['nev']), int(rec['nq']), int(rec['kept']))
+    return same and d <= rel, d
+
+
+def sweep(rates, seed, graphs, names, refs=None, jobs_for=None, progress=False):
+    """one stream per arrival rate, from one library instance; each job checked against refs when given.
+    Returns one report per rate (with the per-job records)"""
+    out = []
+    for rate in rates:
+        n = jobs_for(rate)
+        stream = make_stream(seed, rate, n, graphs)
+        res = [None]*n
+        rep = serve(stream, res, progress=progress)
+        jobs, worst, bad, checked = [], 0.0, 0, 0
+        for (a, g, h, N, eps, top), r in zip(stream, res):
+            j = {'graph': names[g], 'h': h, 'N': N, 'eps': eps, 'top': top, 'arrival_s': round(a, 4),
+                 'S_h': r['S'][0], 'nev': r['nev'], 'nq': r['nq'], 'kept': r['kept_top']}
+            rec = refs.get((names[g], h, N, eps)) if refs else None
+            if rec is not None:
+                ok, d = check_job(r, rec)
+                checked += 1
+                bad += not ok
+                worst = max(worst, d)
+                j['matches_reference'] = ok
+            jobs.append(j)
+        rep.update(rate=rate, seed=seed)
+        if refs is not None:
+            rep['reference_check'] = {'jobs_checked': checked, 'mismatches': bad, 'max_rel_value_diff': worst,
+                                      'all_jobs_match_reference': checked == n and bad == 0}
+        rep['per_job'] = jobs
+        out.append(rep)
+        print('rate %g/s: %d jobs, %.3g points/s, p50 %.3f s, p99 %.3f s%s' % (
+            rate, n, rep['points_per_s'], rep['latency_p50_s'], rep['latency_p99_s'],
+            '' if refs is None else ', %d/%d checked, %d mismatches' % (checked, n, bad)), file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--rate', type=float, default=10.0)
@@ -122,14 +189,29 @@ def main():
     ap.add_argument('--graphs', default='all2,all3')
     ap.add_argument('--workers', type=int, default=16)
     ap.add_argument('--progress', action='store_true', help='one line per finished job on stderr')
+    ap.add_argument('--sweep', default=None, help='comma-separated arrival rates (jobs/s), one stream each, '
+                    'from one library instance; one JSON line per rate')
+    ap.add_argument('--window', type=float, default=15.0,
+                    help='with --sweep: the jobs of a rate are rate x window, between 20 and --jobs')
+    ap.add_argument('--check', default=None, help='folder of the reference\'s "#R" result records '
+                    '(tests/golden/fixtures): every job is checked against its record')
     a = ap.parse_args()
     sfx.init(0)
     sfx.engine_start(a.workers, 0)
     graphs = sfx.select(*a.graphs.split(','))
+    if a.sweep:
+        names = [g['name'] for g in sfx.graphs()]
+        refs = reference_records(a.check) if a.check else None
+        reps = sweep([float(x) for x in a.sweep.split(',')], a.seed, graphs, names, refs,
+                     lambda r: int(max(20, min(a.jobs, round(r*a.window)))), a.progress)
+        for rep in reps:
+            rep.update(workers=a.workers)
+            print(json.dumps(rep))
+        return 0 if refs is None or all(r['reference_check']['all_jobs_match_reference'] for r in reps) else 1
     rep = serve(make_stream(a.seed, a.rate, a.jobs, graphs), progress=a.progress)
     rep.update(rate=a.rate, seed=a.seed, workers=a.workers)
     print(json.dumps(rep))
 
 
 if __name__ == '__main__':
-    main()
+    sys.exit(main())

@@ -33,7 +33,7 @@ class Opts(C.Structure):
                 ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
                 ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int),
                 ('box_scan', C.c_int), ('abcheck', C.c_int),
-                ('growskip', C.c_int)]
+                ('growskip', C.c_int), ('graph_capture', C.c_int)]
 
 
 class Result(C.Structure):

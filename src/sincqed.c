@@ -205,6 +205,7 @@ int main(int argc, char **argv){
     o.box_scan = envi("SINCQED_GROW", 1) == 0;            /* main.c:183: full-box scans instead of the grow */
     o.abcheck = envi("SINCQED_ABCHECK", 0);                 /* main.c:179 */
     o.growskip = envi("SINCQED_GROWSKIP", o.growskip);      /* sum.c:334 */
+    o.graph_capture = envi("SFX_GRAPH_CAPTURE", 0);        /* no counterpart: one HIP graph per refinement level */
     o.ckpt_every = envd("SINCQED_CKPT", 600);              /* checkpoint.c:9, main.c:184 */
     { const char *d = getenv("SINCQED_CKPT_DIR"); if(d) snprintf(o.ckpt_dir, sizeof o.ckpt_dir, "%s", d); }
     int ng = sfx_graph_count();

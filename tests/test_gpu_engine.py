@@ -106,3 +106,28 @@ def test_mix_with_box_scans_and_all_dd(sfx):
         assert _same(a, m), (g, kw, a['S'], m['S'])
         assert (a['esok'], a['es1'], a['es2']) == (m['esok'], m['es1'], m['es2']), (g, kw)
     sfx.engine_stop()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps,kw', [('g2_01', 1.0, 97, 1e-12, {}), ('v3_03', 0.9, 109, 1e-12, {}),
+                                             ('g3_07', 1.0, 97, 1e-10, {'block_points': 1 << 14})])
+def test_graph_capture_same_results(sfx, name, h, N, eps, kw):
+    """a refinement level's block chain run as one captured HIP graph gives the directly launched result bit
+    for bit (the last case: levels of many blocks)"""
+    a = sfx.hsum(name, h, N, eps, **kw)
+    b = sfx.hsum(name, h, N, eps, graph_capture=1, **kw)
+    assert _same(a, b), (name, a['S'], b['S'])
+
+
+@pytest.mark.gpu
+def test_graph_capture_on_engine_workers(sfx):
+    """captures on several engine workers at once (thread-local capture on each worker's stream) give the
+    results the jobs give alone"""
+    alone = [sfx.hsum(g, h, N, eps) for g, h, N, eps in MIX]
+    sfx.engine_stop()
+    sfx.engine_start(4, 0)
+    jobs = [sfx.submit(g, h, N, eps, graph_capture=1) for g, h, N, eps in MIX]
+    got = [sfx.wait(j, timeout=600)[0] for j in jobs]
+    sfx.engine_stop()
+    for (g, h, N, eps), a, m in zip(MIX, alone, got):
+        assert _same(a, m), (g, h, N, eps, a['S'], m['S'])
