@@ -87,37 +87,16 @@ __device__ __forceinline__ double lattice_node(const double *__restrict__ EX, in
 // code and the register pressure drops), 1 every point computes them, 2 chosen per wave.
 constexpr int MODE_NODES = 0, MODE_LIST = 1;
 
-// the body of the evaluation kernels (k_eval for node lists, k_eval_list for a block's cells)
+// the integrand at one point and the reference's per-point tests: value, flag byte, and the slot appended to
+// the double-double list when flagged (every lane of the wave comes here: the append is wave-wide)
 template<class G, int MODE, int AB>
-__device__ __forceinline__ void eval_points(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list,
-                                            const uint32_t *__restrict__ cnt, long n, int st, const sfx_eval_params_t &P,
-                                            double *__restrict__ vout, uint8_t *__restrict__ flout,
-                                            uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+__device__ __forceinline__ void point_out(bool in, bool valid, bool cls, const double *sf, double w, const int *k, int st, long o,
+                                          const sfx_eval_params_t &P, double *__restrict__ vout, uint8_t *__restrict__ flout,
+                                          uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
     constexpr int D = G::D;
-    long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
-    if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
-    if(blockIdx.x*(long)blockDim.x >= n) return;               // the whole block is past the list
-    // the job's table e^{kh} (2K+1 <= 255 values) staged in LDS: the s lookups of every point read it
-    __shared__ double sEX[256];
-    for(int j = threadIdx.x; j < 2*P.K + 1; j += blockDim.x) sEX[j] = P.EX[j];
-    __syncthreads();
-    bool in = i < n;
-    int k[D]; unsigned cfl = 0; bool valid = false; long o = i;
-    if(in){
-        if(MODE == MODE_LIST){
-            long c = list[i >> D]; int b = (int)(i & ((1 << D) - 1));
-            o = (c << D) | b;
-            valid = point_coords<D, true>(pts, o, st, P.K, k, &cfl);
-        } else valid = point_coords<D, false>(pts, i, st, P.K, k, &cfl);
-    }
-    // the skip class: ABSKIP and the parent's skip bit (cells), or a node's nonzero flag byte (the grow's
-    // skip generations at top = 0, sumkey.h:165-168); with ABCHECK its points are evaluated and tested in
-    // full (eval.c:35, 57) and the launch for that class uses AB = 1
-    const bool cls = valid && P.abskip && (MODE == MODE_NODES ? cfl != 0u : (cfl & 1u) != 0u);
     bool skip = cls && !P.abcheck;
     double v = 0, rab = 0; unsigned fl = 0;
     if(valid){
-        double sf[D]; double w = lattice_node<D>(sEX, P.K, P.H, k, sf);
         double ab = 0, f;
         if(AB == 0) f = G::template f<double>(sf, &ab, 0);
         else if(AB == 1) f = G::template f<double>(sf, &ab, 1);
@@ -160,6 +139,100 @@ __device__ __forceinline__ void eval_points(const sfx_cell_t *__restrict__ pts, 
     }
 }
 
+// the body of the evaluation kernels for node lists, and for the cell lists of integrals with D < 6
+template<class G, int MODE, int AB>
+__device__ __forceinline__ void eval_points(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list,
+                                            const uint32_t *__restrict__ cnt, long n, int st, const sfx_eval_params_t &P,
+                                            double *__restrict__ vout, uint8_t *__restrict__ flout,
+                                            uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+    constexpr int D = G::D;
+    long i = blockIdx.x*(long)blockDim.x + threadIdx.x;
+    if(MODE == MODE_LIST){ long m = (long)*cnt << D; if(m < n) n = m; }
+    if(blockIdx.x*(long)blockDim.x >= n) return;               // the whole block is past the list
+    // the job's table e^{kh} (2K+1 <= 255 values) staged in LDS: the s lookups of every point read it
+    __shared__ double sEX[256];
+    for(int j = threadIdx.x; j < 2*P.K + 1; j += blockDim.x) sEX[j] = P.EX[j];
+    __syncthreads();
+    bool in = i < n;
+    int k[D]; unsigned cfl = 0; bool valid = false; long o = i;
+    if(in){
+        if(MODE == MODE_LIST){
+            long c = list[i >> D]; int b = (int)(i & ((1 << D) - 1));
+            o = (c << D) | b;
+            valid = point_coords<D, true>(pts, o, st, P.K, k, &cfl);
+        } else valid = point_coords<D, false>(pts, i, st, P.K, k, &cfl);
+    }
+    // the skip class: ABSKIP and the parent's skip bit (cells), or a node's nonzero flag byte (the grow's
+    // skip generations at top = 0, sumkey.h:165-168); with ABCHECK its points are evaluated and tested in
+    // full (eval.c:35, 57) and the launch for that class uses AB = 1
+    const bool cls = valid && P.abskip && (MODE == MODE_NODES ? cfl != 0u : (cfl & 1u) != 0u);
+    double sf[D], w = 0;
+    if(valid) w = lattice_node<D>(sEX, P.K, P.H, k, sf);
+    point_out<G, MODE, AB>(in, valid, cls, sf, w, k, st, o, P, vout, flout, ddidx, ddcnt);
+}
+
+// Cell lists of the integrals with D >= 6: the 2^D children of a cell fill whole waves, so each wave takes
+// 64-child slices of one cell ("tasks", SFX_LIST_TASKS of them per wave, strided over the block's waves;
+// a per-graph build choice).
+// The slice's cell is wave-uniform -- its list entry and coordinates come through scalar loads -- and a
+// lane's child differs from the cell in the lane's own index bits (child bits 0-5) and the slice's (bits
+// 6 and up): child b sits at k_d = cell_d - st for every set bit d (reference sum.c:405-408).  A lane's
+// table address is the cell's minus a per-lane constant (st or 0 per dimension, fixed for the launch), so
+// the coordinates cost one integer operation per dimension; a dimension whose shifted side leaves the box
+// (cell_d - st < -K, uniform) marks those children invalid (value 0, flag VALID clear), and their lookups
+// land in the table's padding instead.  s, w and f are the same operations on the same values as
+// eval_points: the results are bit-identical.
+#ifndef SFX_LIST_TASKS
+#define SFX_LIST_TASKS 1          // more than one: the loop's hoisted invariants raise the register count
+#endif
+constexpr int SFX_EX_PAD = 128;        // table entries below k = -K: the lookups of invalid children (st <= 128)
+
+template<class G, int AB>
+__device__ __forceinline__ void eval_list_waves(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list,
+                                                const uint32_t *__restrict__ cnt, long n, int st, const sfx_eval_params_t &P,
+                                                double *__restrict__ vout, uint8_t *__restrict__ flout,
+                                                uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
+    constexpr int D = G::D, SH = D - 6;
+    static_assert(D >= 6, "one cell per wave needs 2^D >= 64");
+    const int nwb = SFX_BLOCK/64;
+    { long m = (long)*cnt << D; if(m < n) n = m; }
+    const int ntask = (int)(n >> 6);
+    const int tb = blockIdx.x*nwb*SFX_LIST_TASKS;               // the block's first task
+    if(tb >= ntask) return;
+    __shared__ double sEX[SFX_EX_PAD + 256];
+    for(int j = threadIdx.x; j < 2*P.K + 1; j += blockDim.x) sEX[SFX_EX_PAD + j] = P.EX[j];
+    __syncthreads();
+    const int lane = __lane_id();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // per lane and dimension d < 6: the byte offset subtracted from the cell's table address
+    int sub[6];
+#pragma unroll
+    for(int d = 0; d < 6; d++) sub[d] = ((lane >> d) & 1) ? st*8 : 0;
+    for(int j = 0; j < SFX_LIST_TASKS; j++){
+        const int t = tb + wv + j*nwb;
+        if(t >= ntask) break;
+        const uint32_t c = list[t >> SH];
+        const sfx_cell_t cell = pts[c];
+        const int bh = (t & ((1 << SH) - 1)) << 6;                // the slice's child bits 6 and up
+        const long o = ((long)c << D) | bh | lane;
+        bool valid = true;
+        double sf[D], w = 1;
+#pragma unroll
+        for(int d = 0; d < D; d++){
+            const int ks = cell.k[d];
+            const bool out = ks - st < -P.K;                       // the shifted side leaves the box
+            const int a0 = (SFX_EX_PAD + ks + P.K)*8;              // the cell's own table address (bytes)
+            int a;
+            if(d < 6){ a = a0 - sub[d < 6 ? d : 0]; valid = valid && !(out && ((lane >> d) & 1)); }
+            else { const bool bit = (bh >> d) & 1; a = bit ? a0 - st*8 : a0; valid = valid && !(out && bit); }
+            const double s = *(const double*)((const char*)sEX + a);
+            sf[d] = s; w *= s*P.H;                                 // lattice.c:30-34, index order
+        }
+        const bool cls = valid && P.abskip && (cell.fl & 1u) != 0u;
+        point_out<G, MODE_LIST, AB>(true, valid, cls, sf, w, nullptr, st, o, P, vout, flout, ddidx, ddcnt);
+    }
+}
+
 // Occupancy of the refinement kernels, chosen per graph and class by measurement (sincfeyn_amd/tune.py
 // writes the table that build.py turns into -DSFX_LIST_WAVES0=n / -DSFX_LIST_WAVES1=n for the graph's
 // translation unit): at least n waves per SIMD, i.e. at most 512/n VGPRs; 1 leaves the compiler's choice.
@@ -183,7 +256,8 @@ __global__ void __launch_bounds__(SFX_BLOCK) SFX_LIST_ATTR
 k_eval_list(const sfx_cell_t *__restrict__ pts, const uint32_t *__restrict__ list, const uint32_t *__restrict__ cnt,
             long n, int st, sfx_eval_params_t P, double *__restrict__ vout, uint8_t *__restrict__ flout,
             uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
-    eval_points<G, MODE_LIST, AB>(pts, list, cnt, n, st, P, vout, flout, ddidx, ddcnt);
+    if constexpr(G::D >= 6) eval_list_waves<G, AB>(pts, list, cnt, n, st, P, vout, flout, ddidx, ddcnt);
+    else eval_points<G, MODE_LIST, AB>(pts, list, cnt, n, st, P, vout, flout, ddidx, ddcnt);
 }
 
 // the high-precision recompute of points idx[0 .. *cnt-1] (grid-stride, so the launch does not depend on
@@ -326,6 +400,12 @@ static hipError_t sfx_eval_##NAME(hipStream_t s, int mode, int ab, const sfx_cel
                                   uint8_t *fl, uint32_t *ddidx, uint32_t *ddcnt){                            \
     if(n <= 0) return hipSuccess;                                                                            \
     dim3 g((unsigned)((n + SFX_BLOCK - 1)/SFX_BLOCK)), b(SFX_BLOCK);                                         \
+    if(mode != sfx::MODE_NODES && (D) >= 6){                                                                 \
+        /* 64-child slices, SFX_LIST_TASKS per wave (eval_list_waves) */                                     \
+        if(st > sfx::SFX_EX_PAD) return hipErrorInvalidValue;                                                \
+        const long per = (SFX_BLOCK/64)*SFX_LIST_TASKS;                                                       \
+        g = dim3((unsigned)(((n >> 6) + per - 1)/per));                                                      \
+    }                                                                                                        \
     if(mode == sfx::MODE_NODES)                                                                              \
         hipLaunchKernelGGL((sfx::k_eval<G, sfx::MODE_NODES, 2>), g, b, 0, s, pts, list, cnt, n, st, *P, v, fl, ddidx, ddcnt); \
     else if(ab == 0)                                                                                         \
