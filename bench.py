@@ -143,7 +143,8 @@ def check_reference(res, fx):
         bad += not (same and rel <= 1e-12)
     return {'jobs_checked': n, 'jobs_total': len(res), 'all_match': n > 0 and bad == 0, 'mismatches': bad,
             'max_rel_value_diff': worst, 'against': 'tests/golden/fixtures/nc_*_eps1e-12.txt (reference built '
-            'without FP contraction, SINCQED_SYM=0): evaluation, fallback and kept counts exact, S_h within 1e-12'}
+            'without FP contraction, SINCQED_SYM=0; the counts of refgpu_*.txt, the reference driver\'s GPU build, '
+            'where one is committed): evaluation, fallback and kept counts exact, S_h within 1e-12'}
 
 
 def roofline(S, opc):
