@@ -8,6 +8,9 @@ semantics of the reference's GPU path, sum.c:571):
   cfg2   all3            h 0.7  N 139  eps 1e-12            (41 integrals)
   cfg3   all2 + all3     h 1.0 / 0.8 / 0.7, N 97 / 121 / 139, eps 1e-12   (cfg2 covers all3 at 0.7)
   serve  all2 + all3     h 0.9 N 109 eps 1e-12, and h 1.0 / 0.9 / 0.8 / 0.7 at eps 1e-10
+  stream the (graph, h, eps) jobs of the serving sweep (sincfeyn_amd/serve.py --sweep 1,2,5,10,20,50
+         --jobs 150 --window 15 --seed 7) that cfg2 and cfg3 do not already hold: the seeded streams are
+         drawn here exactly as the sweep draws them (graphs all2 + all3 in the library's registry order)
 for two builds:
   nc     sincqed_ref_nc  the reference without FP contraction (its GPU path's -fmad=false arithmetic):
                          evaluation, fallback and kept counts must equal the device's exactly
@@ -51,6 +54,25 @@ def default_top(h, ts=3.0):
     return top
 
 
+SWEEP = dict(seed=7, rates=(1, 2, 5, 10, 20, 50), jobs=150, window=15.0)
+
+
+def stream_jobs():
+    """{(h, eps): [graph]} of the serving sweep's streams (serve.sweep: one stream per rate, rate x window
+    jobs, at least 20 and at most `jobs`)"""
+    sys.path.insert(0, ROOT)
+    from sincfeyn_amd import serve
+    out = {}
+    names = ALL2 + ALL3
+    for rate in SWEEP['rates']:
+        n = int(max(20, min(SWEEP['jobs'], round(rate*SWEEP['window']))))
+        for a, g, h, N, eps, top in serve.make_stream(SWEEP['seed'], rate, n, list(range(len(names)))):
+            lst = out.setdefault((h, eps), [])
+            if names[g] not in lst:
+                lst.append(names[g])
+    return out
+
+
 def sets(names):
     out = []
     if 'cfg2' in names:
@@ -60,6 +82,12 @@ def sets(names):
     if 'serve' in names:
         out.append((0.9, 1e-12, ALL2 + ALL3))
         out += [(h, 1e-10, ALL2 + ALL3) for h in (1.0, 0.9, 0.8, 0.7)]
+    if 'stream' in names:
+        for (h, eps), graphs in sorted(stream_jobs().items()):
+            if eps == 1e-12 and h in (1.0, 0.8, 0.7):
+                graphs = [g for g in graphs if g not in ALL3 + ALL2]      # held by cfg2 and cfg3
+            if graphs:
+                out.append((h, eps, graphs))
     return out
 
 
