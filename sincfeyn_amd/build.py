@@ -78,6 +78,10 @@ VARIANTS = {
     'burr': ['-mllvm', '-pre-RA-sched=list-burr'],
     'burr_w3': ['-mllvm', '-pre-RA-sched=list-burr', '-DSFX_LIST_WAVES0=3', '-DSFX_LIST_WAVES1=3'],
     'burr_w4': ['-mllvm', '-pre-RA-sched=list-burr', '-DSFX_LIST_WAVES0=4', '-DSFX_LIST_WAVES1=3'],
+    # four 64-child slices per wave in the D >= 6 cell-list kernels (points per thread)
+    't4': ['-DSFX_LIST_TASKS=4'],
+    'w3_t4': ['-DSFX_LIST_WAVES0=3', '-DSFX_LIST_WAVES1=3', '-DSFX_LIST_TASKS=4'],
+    'burr_w3_t4': ['-mllvm', '-pre-RA-sched=list-burr', '-DSFX_LIST_WAVES0=3', '-DSFX_LIST_WAVES1=3', '-DSFX_LIST_TASKS=4'],
 }
 
 

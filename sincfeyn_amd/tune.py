@@ -2,9 +2,11 @@
 
 The level kernels (k_eval_list<G, AB>) of a graph are straight-line FP64 code whose speed depends on how
 many waves share a SIMD (set by their register count, 512/VGPRs at most) and on the instruction
-scheduler; the best choice differs between graphs.  Each candidate in build.VARIANTS (the compiler's own
-register budget, or at least 3 or 4 waves per SIMD; the default or the bottom-up register-reducing
-scheduler) is built as a trial library of the graphs, each trial is timed on the device, and the fastest
+scheduler, and for the D >= 6 cell lists on how many 64-child slices a wave evaluates (points per thread:
+more amortise a wave's start, but the loop can raise the register count); the best choice differs between
+graphs.  Each candidate in build.VARIANTS (the compiler's own register budget, or at least 3 or 4 waves per
+SIMD; the default or the bottom-up register-reducing scheduler; one or four slices per wave) is built as a
+trial library of the graphs, each trial is timed on the device, and the fastest
 variant of each graph goes into sincfeyn_amd/tune_table.json, which build.py applies.  Every variant is the
 same instructions on the same operands: the trials' results are checked to be bit-identical.
 
