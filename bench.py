@@ -316,6 +316,21 @@ def main():
                              'integral; g3_lbl left out: its smallest job here takes minutes on the CPU)' % (
                                  ' '.join('%s@h%g' % j for j in CPU_SAMPLE), ev_c, w_c, 100.0*ev_c/ev*a.steps/SLICES if ev else 0)}
             log('cpu sample: %d points in %.2f s' % (ev_c, w_c))
+            # beside the timed sample: the reference's own time for every job of the workload, as recorded in
+            # the committed fixtures (one process per graph on one host thread, g3_lbl on four, in the machine
+            # that wrote them -- not this one): the whole workload's CPU cost, g3_lbl included
+            fx = fixtures()
+            have = [(name, fx[(name, h)]) for gi, name, D, h, N, top in jobs if 't' in fx.get((name, h), {})]
+            if have:
+                ts = sum(float(r['t'])*(4 if name == 'g3_lbl' else 1) for name, r in have)
+                pts = sum(int(r['nev']) for name, r in have)
+                cpu['workload_from_fixtures'] = {
+                    'jobs': len(have), 'of': len(jobs), 'points': pts, 'thread_seconds': ts,
+                    'points_per_thread_second': pts/ts if ts else None,
+                    'note': 'the reference build without FP contraction, SINCQED_SYM=0, one process per job on one '
+                            'host thread (g3_lbl on four) in the machine that wrote tests/golden/fixtures '
+                            '(make_fixtures.py), several jobs at once beside other work: the thread-seconds are '
+                            'an upper bound of the CPU cost'}
     out = {
         'metric': 'Sinc-sum points/s batched vs per-graph; p50/p99 job latency at <=1e-12 rel',
         'value': ev/dt, 'unit': 'points/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
