@@ -16,6 +16,9 @@ for two builds:
                          evaluation, fallback and kept counts must equal the device's exactly
   ref    sincqed_ref     the reference's default build (FMA contraction): values within 1e-12
 
+(g3_lbl's counts on the device follow the reference's GPU build, not its CPU build: tests/golden/make_refgpu.sh
+writes that build's records for it, refgpu_*.txt.)
+
 Each graph runs as its own process (several at once, each on one or a few OpenMP threads: the
 reference's refinement sum is serial, so processes scale where threads do not).  Records are appended to
 tests/golden/fixtures/<build>_h<h>_N<N>_eps<eps>.txt as they finish; graphs already there are skipped,
