@@ -276,6 +276,10 @@ def main():
     ev_pg = run_per_graph(S, jobs, lat_pg, per_graph)
     dt_pg = time.perf_counter() - t0
     log('per-graph: %d points in %.2f s' % (ev_pg, dt_pg))
+    # where a job's time goes on the per-graph path: the coarsest level (grow, tiers), the dilation, the
+    # refinement levels (each job's own timers, summed over the jobs)
+    stage_s = {k: round(sum(r[f] for r in per_graph.values()), 3)
+               for k, f in (('coarsest_level', 't_grow'), ('dilation', 't_dilate'), ('refinement', 't_refine'))}
     def seq(a_, b_):        # S_h, S_2h, S_4h bit for bit (NaN where a lattice has no such level)
         return all(x == y or (math.isnan(x) and math.isnan(y)) for x, y in zip(a_, b_))
     same = all(seq(r['S'], per_graph[k]['S']) and r['nev'] == per_graph[k]['nev'] and r['nq'] == per_graph[k]['nq']
@@ -344,7 +348,7 @@ def main():
                    'graph_capture': a.graph_capture},
         'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat),
                       'note': 'submit -> result with every timed step queued at once'},
-        'per_graph': {'value': ev_pg/dt_pg, 'wall_s': dt_pg, 'jobs': len(jobs),
+        'per_graph': {'value': ev_pg/dt_pg, 'wall_s': dt_pg, 'jobs': len(jobs), 'stage_s': stage_s,
                       'latency_s': {'p50': percentile(lat_pg, 50), 'p99': percentile(lat_pg, 99)}},
         'batched_over_per_graph': (ev/dt)/(ev_pg/dt_pg),
         'batched_results_identical_to_per_graph': same,
