@@ -1,12 +1,12 @@
 // sfx_integrand.hpp -- included by every emitted integrand translation unit (k_<graph>.hip).
 //
 // It instantiates, for one graph, the device kernels that evaluate the integrand at lattice points:
-//   k_eval<G, MODE, AB>  one thread per point: s = e^{k h} from the job's table, the weight
-//                     w = prod(s_i h) in index order (reference driver/lattice.c:30-34), f in FP64, and the
-//                     reference's per-point tests (driver/eval.c:33-53): the value f w (0 if not finite),
-//                     the "near" flag and the fallback flag.  MODE_LIST: the points are the children
-//                     b = 0 .. 2^D-1 of listed cells c, at k = cell - st b (reference sum.c:405-408); a
-//                     child below the box is not a lattice node (flag VALID clear, value 0).
+//   k_eval<G, MODE_NODES, 2>, k_eval_list<G, AB>  one thread per point: s = e^{k h} from the job's table,
+//                     the weight w = prod(s_i h) in index order (reference driver/lattice.c:30-34), f in FP64,
+//                     and the reference's per-point tests (driver/eval.c:33-53): the value f w (0 if not
+//                     finite), the "near" flag and the fallback flag.  k_eval_list: the points are the
+//                     children b = 0 .. 2^D-1 of listed cells c, at k = cell - st b (reference
+//                     sum.c:405-408); a child below the box is not a lattice node (flag VALID clear, value 0).
 //   k_dd<G, CELLS>    one thread per flagged point (an index list): the same point in double-double,
 //                     then the reference's qvalue (eval.c:25-30): value, eps^2 term of the round column,
 //                     and the "unresolved" mark.
