@@ -111,13 +111,35 @@ def cost(name, h, eps):
     return c*(30 if name == 'g3_lbl' else 1)
 
 
+def dedupe():
+    """keep the first record (and its #X lines) of each graph in every fixture file: two runs that overlapped
+    write the same graph twice (identical but for the time)"""
+    import glob
+    for p in sorted(glob.glob(os.path.join(OUT, '*.txt'))):
+        out, seen, keep = [], set(), True
+        for line in open(p):
+            if line.startswith('#R '):
+                g = line.split()[1]
+                keep = g not in seen
+                seen.add(g)
+            elif not line.startswith('#X '):
+                keep = True
+            if keep:
+                out.append(line)
+        with open(p, 'w') as f:
+            f.writelines(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--builds', default='nc,ref')
     ap.add_argument('--sets', default='cfg2,cfg3,serve')
     ap.add_argument('--threads', type=int, default=os.cpu_count() or 8, help='total OpenMP threads in use at once')
     ap.add_argument('--lbl-threads', type=int, default=4, help='threads of each g3_lbl run')
+    ap.add_argument('--dedupe', action='store_true', help='only drop repeated records from the fixture files')
     a = ap.parse_args(argv)
+    if a.dedupe:
+        return dedupe()
     os.makedirs(OUT, exist_ok=True)
     tasks = []
     for b in a.builds.split(','):
