@@ -132,4 +132,6 @@ def test_serving_sweep_stream_vs_reference(sfx):
     finally:
         sfx.engine_stop()
     print({k: v for k, v in rep.items() if k != 'per_job'}, flush=True)
-    assert rep['reference_check']['all_jobs_match_reference'], [j for j in rep['per_job'] if not j.get('matches_reference')]
+    # every job that has a record matches it (test_fixtures_cover_both_workloads asserts that they all have one)
+    chk = rep['reference_check']
+    assert chk['jobs_checked'] > 0 and chk['mismatches'] == 0, [j for j in rep['per_job'] if j.get('matches_reference') is False]
