@@ -51,6 +51,21 @@ def records(build, h, eps=EPS):
     return out
 
 
+def extrapolations(build, h, eps=EPS):
+    """{graph: (eps*10 change, eps*100 further change)} of the "#X" lines of one fixture file: the reference's
+    eps-extrapolation line (main.c), printed to four significant digits from the shadow-tier sums"""
+    p = os.path.join(FIX, '%s_h%g_N%d_eps%g.txt' % (build, h, default_N(h), eps))
+    out = {}
+    if os.path.exists(p):
+        for line in open(p):
+            if line.startswith('#X ') and 'eps*10 would change the value by' in line:
+                t = line.split()
+                a = line.split('change the value by ')[1].split(',')[0]
+                b = line.split('by a further ')[1].split()[0]
+                out[t[1]] = (float(a), float(b))
+    return out
+
+
 def test_fixtures_cover_both_workloads():
     """the committed fixtures hold every job of both workloads for the reference built without contraction,
     and the reference GPU build's record of g3_lbl at each h (tests/golden/make_refgpu.sh)"""
@@ -95,6 +110,32 @@ def test_jobs_vs_reference_nc(device, h):
         if counts != want or max(rel) > REL or _rel(r['ref'], float(f['ref'])) > 1e-13:
             bad.append((g, counts, want, rel))
     assert not bad, bad
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('h', [0.7, 1.0, 0.8])
+def test_shadow_tiers_vs_reference_nc(sfx, device, h):
+    """the shadow tiers (the kept points the eps*10 and eps*100 floods reach from z = 0) through the sums
+    they select: the reference's eps-extrapolation line, -w (S_h - es1) and -w (es1 - es2), agrees with the
+    device's to its four printed digits for every job (half a unit of the fourth digit, 5e-4 relative,
+    plus the values' own 1e-12)"""
+    X = extrapolations('nc', h)
+    info = {g['name']: g for g in sfx.graphs()}
+    bad, n = [], 0
+    for g, hh in JOBS:
+        if hh != h or g not in X:
+            continue
+        r, w = device[(g, h)], info[g]['weight']
+        if not r['esok']:
+            bad.append((g, 'no tiers'))
+            continue
+        d1, d2 = -w*(r['S'][0] - r['es1']), -w*(r['es1'] - r['es2'])
+        for dev, ref in zip((d1, d2), X[g]):
+            if abs(dev - ref) > 5.01e-4*abs(ref) + 1e-12*abs(r['S'][0]*w):
+                bad.append((g, dev, ref))
+        n += 1
+    print('h=%g: %d jobs, eps-extrapolation terms as the reference prints them' % (h, n), flush=True)
+    assert n > 0 and not bad, bad
 
 
 @pytest.mark.gpu
