@@ -53,6 +53,8 @@ constexpr int TILE_THREADS = 256;
 constexpr int TILE_ITEMS = 32;                      // items per thread
 constexpr long TILE = (long)TILE_THREADS*TILE_ITEMS;
 
+// exclusive scan of the block counts cnt[0 .. nb) into off[], their total into *total (one workgroup)
+hipError_t scan_blocks(hipStream_t s, const uint32_t *cnt, long nb, uint32_t *off, uint32_t *total);
 // ordered compaction: out[0..*count-1] = the indices i < n with (fl[i] & mask) != 0, ascending.
 // Scratch: blk (2 * ceil(n/TILE) + 1 uint32).
 hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask, uint32_t *blk, uint32_t *out, uint32_t *count);

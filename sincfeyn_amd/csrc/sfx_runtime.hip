@@ -114,6 +114,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_scatter_flags(const uint8_t *f
     }
 }
 
+hipError_t scan_blocks(hipStream_t s, const uint32_t *cnt, long nb, uint32_t *off, uint32_t *total){
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, s, cnt, nb, off, total, (const unsigned long long*)nullptr);
+    return hipGetLastError();
+}
+
 hipError_t compact_flags(hipStream_t s, const uint8_t *fl, long n, uint8_t mask, uint32_t *blk, uint32_t *out, uint32_t *count){
     long nb = (n + TILE - 1)/TILE;
     if(nb == 0) return hipMemsetAsync(count, 0, sizeof(uint32_t), s);
