@@ -33,12 +33,12 @@ struct JobBufs {
     DevBuf vfe;                             // a node list's values, round terms and flag bytes, back to back
     DevBuf lsk, lfu, lcnt;                  // a refinement block's cells split by |.|-sum class; counts
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
-    DevBuf fr, nx, gkey, gval, misc, kflag, kidx, keep, kv, src, ckey, cidx, cflag, widx, keep2, kv2, gval2, tr, q1, q2,
+    DevBuf fr, nx, gkey, gval, gclm, misc, kflag, kidx, keep, kv, src, ckey, cflag, widx, keep2, kv2, gval2, tr, q1,
            dkey, skey, skey2, sortt;
     template<class F> void each(F f){
         DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc, &vfe,
-                         &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cidx, &cflag, &widx,
-                         &keep2, &kv2, &gval2, &tr, &q1, &q2, &dkey, &skey, &skey2, &sortt};
+                         &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &gclm, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cflag, &widx,
+                         &keep2, &kv2, &gval2, &tr, &q1, &dkey, &skey, &skey2, &sortt};
         for(DevBuf *b : all) f(*b);
     }
     void release(){ each([](DevBuf &b){ b.release(); }); }

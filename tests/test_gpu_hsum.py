@@ -14,8 +14,8 @@ def _cmp(sfx, oracle, name, h, N, eps, top=None, rel=REL, **kw):
     top = sfx.default_top(h) if top is None else top
     r = sfx.hsum(name, h, N, eps, top, **kw)
     o = oracle.hsum(name, h, N, eps, top, **kw)
-    print(name, h, N, eps, 'device S_h %.17g  oracle %.17g  nev %d/%d nq %d/%d kept %d/%d  t %.3fs' % (
-        r['S'][0], o['S'][0], r['nev'], o['nev'], r['nq'], o['nq'], r['kept_top'], o['kept_top'], r['t']), flush=True)
+    print(name, h, N, eps, 'device S_h %.17g  oracle %.17g  nev %d/%d nq %d/%d kept %d/%d btuns %d/%d  t %.3fs' % (
+        r['S'][0], o['S'][0], r['nev'], o['nev'], r['nq'], o['nq'], r['kept_top'], o['kept_top'], r['btuns'], o['btuns'], r['t']), flush=True)
     assert r['kept_top'] == o['kept_top']
     assert r['nev'] == o['nev']
     assert r['nev_level'] == o['nev_level']
@@ -28,6 +28,8 @@ def _cmp(sfx, oracle, name, h, N, eps, top=None, rel=REL, **kw):
             assert abs(a - b) <= rel*abs(b), (j, a, b)
     assert abs(r['ref'] - o['ref']) <= 1e-13*abs(o['ref'])
     assert r['abskipped'] == o['abskipped']
+    # the grow's candidates that fell below the freed boundary tables (the BTBOUND count, sumkey.h:342,397)
+    assert r['btuns'] == o['btuns'], (r['btuns'], o['btuns'])
     # the shadow-tier sums (the eps-extrapolation line): the same tier sets, so against the oracle of the
     # same arithmetic the sums differ by their summation error only, far below one tier-1 point (|v| >=
     # 10 thr = 1e-3 eps Ref); against a differently rounded build, by what the values' tolerance allows
