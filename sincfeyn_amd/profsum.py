@@ -3,8 +3,9 @@
     python -m sincfeyn_amd.profsum counters OUT.json CSV... [--match REGEX]
         mean, per dispatch, of every counter in the counter-collection CSVs (rocprofv3 --pmc ...
         --output-format csv), per kernel name matching REGEX, with the dispatch count and grid size
-    python -m sincfeyn_amd.profsum stats OUT.md KERNEL_STATS_CSV [--top N]
-        the first N rows of a kernel-stats CSV (rocprofv3 --kernel-trace --stats) as a markdown table
+    python -m sincfeyn_amd.profsum stats OUT.md KERNEL_STATS_CSV [--top N] [--parts]
+        the first N rows of a kernel-stats CSV (rocprofv3 --kernel-trace --stats) as a markdown table; with
+        --parts, short kernel names and first a table of the device time by part of the hierarchical sum
 
 The committed tables are what bench.py reads: the executed FP64 operations per dispatch of its roofline
 kernel (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64) and its HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE),
@@ -44,13 +45,45 @@ def counters(out, files, match):
     print('%d kernels -> %s' % (len(res), out))
 
 
-def stats(out, fn, top):
+def short_name(name):
+    """a kernel's name without its parameter list, namespaces and graph-type decoration:
+    k_eval_list<g3_lbl, 0>, k_dd<g3_21, true>, k_grow_cand, rocprim radix sort"""
+    if 'rocprim' in name:
+        return 'rocprim radix sort' if 'radix_sort' in name else 'rocprim ' + name.split('(')[0][-40:]
+    n = name.split('(')[0] if not name.startswith('void ') else name[5:].split('(')[0]
+    return n.replace('sfx::', '').replace('sfxk_', '').replace('::Graph', '').strip()
+
+
+# the parts of a hierarchical sum, by kernel name (first match wins)
+PARTS = [('refinement evaluation (k_eval_list)', re.compile(r'^k_eval_list<')),
+         ('refinement sums, class split, fallback', re.compile(r'^(k_level_|k_cell_split|k_zero_acc|k_dd<.*, true>)')),
+         ('coarsest level: frontier evaluation and fallback', re.compile(r'^(k_eval<|k_dd<.*, false>)')),
+         ('coarsest level: grow, tiers, dilation', re.compile(r'^(k_grow_|k_front_|k_gmap_|k_tier_|k_kmap_|k_dil_|k_drop_|'
+                                                            r'k_gather_keep|k_scan_u32|k_count_flags|k_scatter_flags|rocprim)')),
+         ('other (copies, fills, counters)', re.compile(r'.'))]
+
+
+def stats(out, fn, top, parts=False):
     with open(fn) as f:
         rows = list(csv.DictReader(f))
-    keys = [k for k in ('Name', 'Calls', 'TotalDurationNs', 'AverageNs', 'Percentage', 'MinNs', 'MaxNs') if rows and k in rows[0]]
-    lines = ['| ' + ' | '.join(keys) + ' |', '|' + '---|'*len(keys)]
-    for r in rows[:top]:
-        lines.append('| ' + ' | '.join(r[k].replace('|', '/') for k in keys) + ' |')
+    lines = []
+    if parts:
+        tot = sum(float(r['TotalDurationNs']) for r in rows)
+        by = collections.OrderedDict((p, 0.0) for p, _ in PARTS)
+        for r in rows:
+            n = short_name(r['Name'])
+            by[next(p for p, rx in PARTS if rx.search(n))] += float(r['TotalDurationNs'])
+        lines += ['Device time of all kernels: %.0f ms.' % (tot/1e6), '', '| part | ms | share |', '|---|---|---|']
+        lines += ['| %s | %.0f | %.1f%% |' % (p, v/1e6, 100*v/tot) for p, v in by.items()]
+        lines += ['', '| kernel | calls | total ms | avg ms | % of device time |', '|---|---|---|---|---|']
+        for r in rows[:top]:
+            lines.append('| `%s` | %s | %.1f | %.4f | %.2f |' % (short_name(r['Name']), r['Calls'], float(r['TotalDurationNs'])/1e6,
+                                                            float(r['AverageNs'])/1e6, float(r['Percentage'])))
+    else:
+        keys = [k for k in ('Name', 'Calls', 'TotalDurationNs', 'AverageNs', 'Percentage', 'MinNs', 'MaxNs') if rows and k in rows[0]]
+        lines += ['| ' + ' | '.join(keys) + ' |', '|' + '---|'*len(keys)]
+        for r in rows[:top]:
+            lines.append('| ' + ' | '.join(r[k].replace('|', '/') for k in keys) + ' |')
     with open(out, 'w') as f:
         f.write('\n'.join(lines) + '\n')
     print('%d rows -> %s' % (min(top, len(rows)), out))
@@ -61,11 +94,12 @@ def main(argv=None):
     sub = ap.add_subparsers(dest='cmd', required=True)
     c = sub.add_parser('counters'); c.add_argument('out'); c.add_argument('files', nargs='+'); c.add_argument('--match', default='.')
     s = sub.add_parser('stats'); s.add_argument('out'); s.add_argument('file'); s.add_argument('--top', type=int, default=40)
+    s.add_argument('--parts', action='store_true')
     a = ap.parse_args(argv)
     if a.cmd == 'counters':
         counters(a.out, a.files, a.match)
     else:
-        stats(a.out, a.file, a.top)
+        stats(a.out, a.file, a.top, a.parts)
 
 
 if __name__ == '__main__':
