@@ -162,38 +162,48 @@ __device__ __forceinline__ void tile_store(LevelPart t, LevelPart *part, unsigne
     }
 }
 
-// The loads of LOAD_BATCH items are issued together before their sums (one memory latency per batch
-// instead of one per item); the items are added in the same order as one at a time.
-constexpr int LOAD_BATCH = 8;
-static_assert(TILE_ITEMS % LOAD_BATCH == 0, "tile items in whole load batches");
+// A thread takes its tile's children four consecutive ones at a time (the flag bytes as one 32-bit load, the
+// values as two 16-byte loads, the four children's cell flag once), QUAD_BATCH quads loaded together before
+// their sums (one memory latency per batch); within a thread the children are added in index order.
+constexpr int QUAD_BATCH = 4;
+static_assert(TILE_ITEMS % (4*QUAD_BATCH) == 0, "tile items in whole quad batches");
+__device__ __forceinline__ void level_item(LevelPart &t, const LevelArgs &a, long i, unsigned f, double v, unsigned cfl){
+    if(!(f & SFX_FL_VALID)) return;
+    const double x = v*a.pw, av = fabs(v);
+    const bool sk = a.abskip && (cfl & 1u);
+    t.nv++; t.nskip += sk ? 1u : 0u;
+    if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; t.nmiss += sk ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
+    dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
+    if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
+    else if(av >= a.thr) t.nkeep++;
+    else dacc_add(t.drop[0], t.drop[1], av*a.pw);
+}
 __global__ void __launch_bounds__(TILE_THREADS) k_level_partials(LevelArgs a){
-    const long base = (long)blockIdx.x*TILE + threadIdx.x;
+    const long base = (long)blockIdx.x*TILE;
     LevelPart t = {};
-    for(int j0 = 0; j0 < TILE_ITEMS; j0 += LOAD_BATCH){
-        if(base + (long)j0*TILE_THREADS >= a.n) break;
-        uint8_t fb[LOAD_BATCH], cb[LOAD_BATCH]; double vb[LOAD_BATCH];
+    for(int q0 = 0; q0 < TILE_ITEMS/4; q0 += QUAD_BATCH){
+        if(base + ((long)q0*TILE_THREADS + threadIdx.x)*4 >= a.n) break;
+        uint32_t fq[QUAD_BATCH]; double2 vq[QUAD_BATCH][2]; uint8_t cq[QUAD_BATCH];
         #pragma unroll
-        for(int k = 0; k < LOAD_BATCH; k++){
-            const long i = base + (long)(j0 + k)*TILE_THREADS;
-            const bool in = i < a.n;
-            fb[k] = in ? a.fl[i] : 0;
-            vb[k] = in ? a.v[i] : 0.0;
-            cb[k] = in ? a.cells[i >> a.D].fl : 0;
+        for(int k = 0; k < QUAD_BATCH; k++){
+            const long i0 = base + ((long)(q0 + k)*TILE_THREADS + threadIdx.x)*4;
+            if(i0 + 4 <= a.n){
+                fq[k] = *(const uint32_t*)(a.fl + i0);
+                vq[k][0] = *(const double2*)(a.v + i0); vq[k][1] = *(const double2*)(a.v + i0 + 2);
+            } else {
+                uint32_t f = 0; double w[4] = {0, 0, 0, 0};
+                for(int j = 0; j < 4; j++) if(i0 + j < a.n){ f |= (uint32_t)a.fl[i0 + j] << (8*j); w[j] = a.v[i0 + j]; }
+                fq[k] = f; vq[k][0] = make_double2(w[0], w[1]); vq[k][1] = make_double2(w[2], w[3]);
+            }
+            cq[k] = i0 < a.n ? a.cells[i0 >> a.D].fl : 0;          // D >= 2: the four children share their cell
         }
         #pragma unroll
-        for(int k = 0; k < LOAD_BATCH; k++){
-            const uint8_t f = fb[k];
-            if(!(f & SFX_FL_VALID)) continue;
-            const long i = base + (long)(j0 + k)*TILE_THREADS;
-            const double v = vb[k], x = v*a.pw, av = fabs(v);
-            const unsigned cfl = cb[k];
-            const bool sk = a.abskip && (cfl & 1u);
-            t.nv++; t.nskip += sk ? 1u : 0u;
-            if(f & SFX_FL_DD){ t.nq++; t.nd += (f & SFX_FL_UNRES) ? 1u : 0u; t.nmiss += sk ? 1u : 0u; dacc_add(t.u[0], t.u[1], a.e2[i]); }
-            dacc_add(t.s[0], t.s[1], x); dacc_add(t.a[0], t.a[1], av);
-            if(a.last){ int te = (int)(cfl >> 1); if(te >= 1) dacc_add(t.b1[0], t.b1[1], x); if(te >= 2) dacc_add(t.b2[0], t.b2[1], x); }
-            else if(av >= a.thr) t.nkeep++;
-            else dacc_add(t.drop[0], t.drop[1], av*a.pw);
+        for(int k = 0; k < QUAD_BATCH; k++){
+            const long i0 = base + ((long)(q0 + k)*TILE_THREADS + threadIdx.x)*4;
+            level_item(t, a, i0, fq[k] & 0xffu, vq[k][0].x, cq[k]);
+            level_item(t, a, i0 + 1, (fq[k] >> 8) & 0xffu, vq[k][0].y, cq[k]);
+            level_item(t, a, i0 + 2, (fq[k] >> 16) & 0xffu, vq[k][1].x, cq[k]);
+            level_item(t, a, i0 + 3, fq[k] >> 24, vq[k][1].y, cq[k]);
         }
     }
     tile_store(t, a.part, a.keepoff);
@@ -282,6 +292,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_level_scatter(LevelArgs a){
 }
 
 hipError_t level_reduce(hipStream_t s, const LevelArgs &a){
+    if(a.D < 2) return hipErrorInvalidValue;                 // k_level_partials reads four children of one cell at once
     long nb = (a.n + TILE - 1)/TILE;
     if(nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_level_partials, dim3((unsigned)nb), dim3(TILE_THREADS), 0, s, a);
