@@ -33,11 +33,11 @@ struct JobBufs {
     DevBuf vfe;                             // a node list's values, round terms and flag bytes, back to back
     DevBuf lsk, lfu, lcnt;                  // a refinement block's cells split by |.|-sum class; counts
     // the coarsest level (sfx_grow.inc): frontiers, map of evaluated points, claims, keep list, tiers, dilation
-    DevBuf fr, nx, gkey, gval, gclm, misc, kflag, kidx, keep, kv, src, ckey, cflag, widx, keep2, kv2, gval2, tr, q1,
+    DevBuf fr, nx, gkey, misc, kflag, kidx, keep, kv, src, ckey, cflag, widx, keep2, kv2, gval2, tr, q1,
            dkey, skey, skey2, sortt;
     template<class F> void each(F f){
         DevBuf *all[] = {&EX, &pts, &v, &fl, &ddblk, &ddidx, &ddcnt, &e2, &nd, &cur, &nxt, &part, &keepoff, &acc, &vfe,
-                         &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &gval, &gclm, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cflag, &widx,
+                         &lsk, &lfu, &lcnt, &fr, &nx, &gkey, &misc, &kflag, &kidx, &keep, &kv, &src, &ckey, &cflag, &widx,
                          &keep2, &kv2, &gval2, &tr, &q1, &dkey, &skey, &skey2, &sortt};
         for(DevBuf *b : all) f(*b);
     }
