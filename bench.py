@@ -24,8 +24,10 @@ Beside it, on the same machine:
                events on the library's stream), algorithmic and executed (committed VALU counters)
   reference    each batched result against the reference's own results (committed fixtures written by
                tests/golden/make_fixtures.py): counts exact, values within 1e-12
+  served       p50/p99 latency of a seeded Poisson stream of (graph, h, eps) jobs at 10 jobs/s through the same
+               engine, each job checked against the reference's result record
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline] [--no-plain]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline] [--no-plain] [--no-serve]
 """
 import argparse
 import glob
@@ -53,6 +55,8 @@ KNAME = {'level_eval': 'k_eval_list<%s, AB=1>', 'level_eval_skip': 'k_eval_list<
 PLAIN_SAMPLE = [('g3_00', 1.0), ('g3_lbl', 1.0), ('g3_01', 0.7), ('v3_03', 0.8), ('v3_p4Sc', 1.0),
                 ('g2_00', 1.0), ('g2_01', 1.0), ('g2_02', 1.0), ('g2_03', 1.0), ('v2_00', 1.0)]
 CPU_SAMPLE = [j for j in PLAIN_SAMPLE if j[0] != 'g3_lbl']
+# the served stream beside the throughput run: the serving sweep's stream at 10 jobs/s (150 jobs, about 15 s)
+SERVE_RATE, SERVE_SEED, SERVE_JOBS = 10.0, 7, 150
 
 
 def percentile(xs, p):
@@ -214,6 +218,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain GPU path sample')
+    ap.add_argument('--no-serve', action='store_true', help='skip the served stream')
     ap.add_argument('--workers', type=int, default=16, help='engine workers of the batched path')
     ap.add_argument('--profile-only', action='store_true',
                     help='only the profiled per-graph pass (one host thread: the form profiled with rocprofv3)')
@@ -285,6 +290,20 @@ def main():
     same = all(seq(r['S'], per_graph[k]['S']) and r['nev'] == per_graph[k]['nev'] and r['nq'] == per_graph[k]['nq']
                for k, r in batched.items())
     ref = check_reference(batched, fixtures())
+    # served latency: the serving sweep's seeded stream at SERVE_RATE jobs/s through the same engine, every job
+    # checked against the reference's result record (sincfeyn_amd/serve.py)
+    served = None
+    if not a.no_serve:
+        from sincfeyn_amd import serve as SV
+        names = [g['name'] for g in S.graphs()]
+        refs = SV.reference_records(os.path.join(ROOT, 'tests', 'golden', 'fixtures'))
+        rep = SV.sweep([SERVE_RATE], SERVE_SEED, S.select('all2', 'all3'), names, refs, lambda r: SERVE_JOBS)[0]
+        served = {k: rep[k] for k in ('rate', 'seed', 'jobs', 'wall_s', 'points_per_s', 'latency_p50_s', 'latency_p99_s',
+                                      'queue_p50_s', 'queue_p99_s', 'reference_check')}
+        served['stream'] = ('Poisson arrivals at %g jobs/s (seed %d): graphs uniform over all2 + all3, h in {1.0, 0.9, 0.8, '
+                            '0.7}, eps in {1e-10, 1e-12}; latency = arrival -> result' % (SERVE_RATE, SERVE_SEED))
+        log('served stream: %d jobs at %g/s, p50 %.3f s, p99 %.3f s' % (rep['jobs'], SERVE_RATE, rep['latency_p50_s'],
+                                                                        rep['latency_p99_s']))
 
     plain = None
     cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
@@ -347,7 +366,9 @@ def main():
                    'path': 'batched', 'workers': a.workers, 'precision': 'FP64 bulk, double-double where flagged',
                    'graph_capture': a.graph_capture},
         'latency_s': {'p50': percentile(lat, 50), 'p99': percentile(lat, 99), 'jobs': len(lat),
-                      'note': 'submit -> result with every timed step queued at once'},
+                      'note': 'submit -> result with every timed step queued at once (a throughput run; the served '
+                              'latencies are in "served")'},
+        'served': served,
         'per_graph': {'value': ev_pg/dt_pg, 'wall_s': dt_pg, 'jobs': len(jobs), 'stage_s': stage_s,
                       'latency_s': {'p50': percentile(lat_pg, 50), 'p99': percentile(lat_pg, 99)}},
         'batched_over_per_graph': (ev/dt)/(ev_pg/dt_pg),
