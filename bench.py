@@ -227,6 +227,8 @@ def main():
     ap.add_argument('--profile-out', default=None, help='write the per-kernel table of the profiled pass as JSON here')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
+    ap.add_argument('--batched-only', action='store_true',
+                    help='only the warmup and timed batched steps (for a kernel trace of the engine\'s timeline)')
     ap.add_argument('--graph-capture', type=int, default=0,
                     help='1: every refinement level\'s block chain runs as one captured HIP graph (sfx_opts_t.graph_capture)')
     a = ap.parse_args()
@@ -245,6 +247,20 @@ def main():
 
     def log(*x):
         print('[bench]', *x, file=sys.stderr, flush=True)
+
+    if a.batched_only:
+        jobs, slices = workload(S)
+        steps = [slices[i % SLICES] for i in range(a.warmup + a.steps)]
+        S.engine_start(a.workers, 0)
+        run_batched(S, steps[:a.warmup], [], {})
+        t0 = time.perf_counter()
+        ev = run_batched(S, steps[a.warmup:], [], {})
+        dt = time.perf_counter() - t0
+        log('batched: %d steps, %d points in %.2f s' % (a.steps, ev, dt))
+        print(json.dumps({'metric': 'Sinc-sum points/s, batched steps only', 'value': ev/dt, 'unit': 'points/s',
+                          'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup, 'wall_s': dt, 'path': 'batched',
+                          'workers': a.workers, 'device': info}))
+        return
 
     # the profiled per-graph pass (kernel durations from HIP events on the library's stream)
     pgraphs = [(gi, S.graphs()[gi]['name'], S.graphs()[gi]['D'], h, N, S.default_top(h))

@@ -85,6 +85,12 @@ VARIANTS = {
 }
 
 
+# every integrand unit: the kernels' leading scalar arguments (point and list addresses, the count's address,
+# n, st) arrive preloaded in SGPRs, so a wave's first load -- the list count -- does not wait for its own
+# address (gfx950 kernarg preloading; the code object keeps the loads for a launcher that does not preload)
+KERNEL_FLAGS = ['-mllvm', '-amdgpu-kernarg-preload-count=16']
+
+
 def unit_flags(src, use_table=True):
     """extra hipcc flags of one translation unit.  The integrands emitted with node_split (their FP64 point
     function as one switch case per forest node) are scheduled bottom-up for register need, which keeps
@@ -151,8 +157,8 @@ def build(force=True, jobs=None, classes=DEFAULT_CLASSES, only=None, libdir=None
 
     def kflags(src):
         if variant is None:
-            return unit_flags(src)
-        return unit_flags(src, use_table=False) or list(VARIANTS[variant])
+            return KERNEL_FLAGS + unit_flags(src)
+        return KERNEL_FLAGS + (unit_flags(src, use_table=False) or list(VARIANTS[variant]))
     units = [(os.path.join(GEN, 'k_%s.hip' % n), os.path.join(objdir, 'k_%s.o' % n), hdrs + [os.path.join(GEN, 'k_%s.inc' % n)] + tdeps)
              for n in names]
     units.append((reg, os.path.join(objdir, 'sfx_registry.o'), hdrs))

@@ -194,32 +194,50 @@ __device__ __forceinline__ void eval_list_waves(const sfx_cell_t *__restrict__ p
                                                 uint32_t *__restrict__ ddidx, uint32_t *__restrict__ ddcnt){
     constexpr int D = G::D, SH = D - 6;
     static_assert(D >= 6, "one cell per wave needs 2^D >= 64");
+    static_assert(sizeof(sfx_cell_t) == 16 && offsetof(sfx_cell_t, fl) == 15, "a cell is read as one uint4");
     const int nwb = SFX_BLOCK/64;
-    { long m = (long)*cnt << D; if(m < n) n = m; }
+    // the count, the table's address and K requested together (the empty asm keeps the compiler from
+    // sinking the last two past the count's test)
+    const uint32_t cn = *cnt;
+    unsigned long long exa = (unsigned long long)P.EX; int K = P.K;
+    asm volatile("" : "+s"(exa), "+s"(K));
+    const __attribute__((address_space(1))) double *EX = (const __attribute__((address_space(1))) double*)exa;
+    { long m = (long)cn << D; if(m < n) n = m; }
     const int ntask = (int)(n >> 6);
     const int tb = blockIdx.x*nwb*SFX_LIST_TASKS;               // the block's first task
     if(tb >= ntask) return;
-    __shared__ double sEX[SFX_EX_PAD + 256];
-    for(int j = threadIdx.x; j < 2*P.K + 1; j += blockDim.x) sEX[SFX_EX_PAD + j] = P.EX[j];
-    __syncthreads();
     const int lane = __lane_id();
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // the table's entry of this thread (2K + 1 <= 255 entries, one per thread) is requested first, then the
+    // wave's first slice -- its list entry and its cell, one 16-byte scalar load -- so the two latencies
+    // overlap; each later slice's cell is requested before the current slice's evaluation
+    static_assert(SFX_BLOCK >= 255, "one table entry per thread");
+    __shared__ double sEX[SFX_EX_PAD + 256];
+    const bool hx = (int)threadIdx.x < 2*K + 1;
+    double ex = 0;
+    if(hx) ex = EX[threadIdx.x];
+    int t = tb + wv;
+    uint32_t c = 0; uint4 raw = make_uint4(0, 0, 0, 0);
+    if(t < ntask){ c = list[t >> SH]; raw = *(const uint4*)(pts + c); }
+    if(hx) sEX[SFX_EX_PAD + threadIdx.x] = ex;
+    __syncthreads();
     // per lane and dimension d < 6: the byte offset subtracted from the cell's table address
     int sub[6];
 #pragma unroll
     for(int d = 0; d < 6; d++) sub[d] = ((lane >> d) & 1) ? st*8 : 0;
-    for(int j = 0; j < SFX_LIST_TASKS; j++){
-        const int t = tb + wv + j*nwb;
+    for(int j = 0; j < SFX_LIST_TASKS; j++, t += nwb){
         if(t >= ntask) break;
-        const uint32_t c = list[t >> SH];
-        const sfx_cell_t cell = pts[c];
+        const uint32_t cc = c; const uint4 cr = raw;
+        if(j + 1 < SFX_LIST_TASKS && t + nwb < ntask){ c = list[(t + nwb) >> SH]; raw = *(const uint4*)(pts + c); }
+        const unsigned cw[4] = {cr.x, cr.y, cr.z, cr.w};
+        const unsigned cfl = cr.w >> 24;                            // sfx_cell_t: k[0..13], pad, fl (byte 15)
         const int bh = (t & ((1 << SH) - 1)) << 6;                // the slice's child bits 6 and up
-        const long o = ((long)c << D) | bh | lane;
+        const long o = ((long)cc << D) | bh | lane;
         bool valid = true;
         double sf[D], w = 1;
 #pragma unroll
         for(int d = 0; d < D; d++){
-            const int ks = cell.k[d];
+            const int ks = (int)(int8_t)(cw[d >> 2] >> (8*(d & 3)));
             const bool out = ks - st < -P.K;                       // the shifted side leaves the box
             const int a0 = (SFX_EX_PAD + ks + P.K)*8;              // the cell's own table address (bytes)
             int a;
@@ -228,7 +246,7 @@ __device__ __forceinline__ void eval_list_waves(const sfx_cell_t *__restrict__ p
             const double s = *(const double*)((const char*)sEX + a);
             sf[d] = s; w *= s*P.H;                                 // lattice.c:30-34, index order
         }
-        const bool cls = valid && P.abskip && (cell.fl & 1u) != 0u;
+        const bool cls = valid && P.abskip && (cfl & 1u) != 0u;
         point_out<G, MODE_LIST, AB>(true, valid, cls, sf, w, nullptr, st, o, P, vout, flout, ddidx, ddcnt);
     }
 }

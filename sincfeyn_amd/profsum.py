@@ -6,6 +6,11 @@
     python -m sincfeyn_amd.profsum stats OUT.md KERNEL_STATS_CSV [--top N] [--parts]
         the first N rows of a kernel-stats CSV (rocprofv3 --kernel-trace --stats) as a markdown table; with
         --parts, short kernel names and first a table of the device time by part of the hierarchical sum
+    python -m sincfeyn_amd.profsum timeline OUT.md KERNEL_TRACE_CSV [--last SECONDS] [--bins N]
+        the device's timeline from a kernel trace (rocprofv3 --kernel-trace): over the last SECONDS of the
+        trace (default all of it), the time some kernel runs, the time an integrand evaluation runs, the
+        idle gaps, the number of kernels running at once, the occupancy per time bin, and the graphs whose
+        kernels run in the last tenth
 
 The committed tables are what bench.py reads: the executed FP64 operations per dispatch of its roofline
 kernel (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64) and its HBM bytes per dispatch (FETCH_SIZE, WRITE_SIZE),
@@ -89,15 +94,100 @@ def stats(out, fn, top, parts=False):
     print('%d rows -> %s' % (min(top, len(rows)), out))
 
 
+EVAL = re.compile(r'^(k_eval_list<|k_eval<|k_dd<|k_qd<)')
+
+
+def timeline(out, fn, last=None, bins=20):
+    ks = []
+    with open(fn) as f:
+        for r in csv.DictReader(f):
+            ks.append((int(r['Start_Timestamp']), int(r['End_Timestamp']), short_name(r['Kernel_Name'])))
+    if not ks:
+        raise SystemExit('no dispatches in ' + fn)
+    t_end = max(e for _, e, _ in ks)
+    t0 = t_end - int(last*1e9) if last else min(s for s, _, _ in ks)
+    ks = [(max(s, t0), e, n) for s, e, n in ks if e > t0]
+    span = t_end - t0
+    # sweep over start and end points: the number of kernels running, and of evaluation kernels among them
+    ev = sorted([(s, 1, EVAL.search(n) is not None) for s, e, n in ks] + [(e, -1, EVAL.search(n) is not None) for s, e, n in ks])
+    run = run_ev = 0
+    busy = ev_busy = 0
+    conc = collections.Counter()
+    gaps = []
+    prev = t0
+    nb = max(1, bins)
+    bb = [0]*nb; be = [0]*nb
+
+    def add(a, b, n, n_ev):
+        nonlocal busy, ev_busy
+        if b <= a:
+            return
+        conc[min(n, 4)] += b - a
+        if n:
+            busy += b - a
+        if n_ev:
+            ev_busy += b - a
+        # per-bin shares
+        i = min(nb - 1, (a - t0)*nb//span)
+        while a < b:
+            hi = t0 + (i + 1)*span//nb
+            c = min(b, hi)
+            if n:
+                bb[i] += c - a
+            if n_ev:
+                be[i] += c - a
+            a = c
+            i = min(nb - 1, i + 1)
+    for t, d, is_ev in ev:
+        if run == 0 and t > prev:
+            gaps.append(t - prev)
+        add(prev, t, run, run_ev)
+        run += d
+        run_ev += d if is_ev else 0
+        prev = t
+    # the graphs of the kernels in the last tenth of the window (device time by graph)
+    tail0 = t_end - span//10
+    tail = collections.Counter()
+    for s, e, n in ks:
+        if e > tail0:
+            m = re.search(r'<(\w+)', n)
+            tail[m.group(1) if m and not n.startswith('rocprim') else '(no graph)'] += e - max(s, tail0)
+    ms = lambda x: x/1e6
+    L = ['Window: %.1f ms (%s).' % (ms(span), 'the last %.3f s of the trace' % last if last else 'the whole trace'),
+         '', '| | ms | share of the window |', '|---|---|---|',
+         '| some kernel running | %.1f | %.1f%% |' % (ms(busy), 100*busy/span),
+         '| an integrand evaluation running (k_eval_list, k_eval, k_dd, k_qd) | %.1f | %.1f%% |' % (ms(ev_busy), 100*ev_busy/span),
+         '| only other kernels running | %.1f | %.1f%% |' % (ms(busy - ev_busy), 100*(busy - ev_busy)/span),
+         '| no kernel running | %.1f | %.1f%% |' % (ms(span - busy), 100*(span - busy)/span),
+         '', 'Kernels running at once (share of the window): ' +
+         ', '.join('%s: %.1f%%' % ('4+' if k == 4 else k, 100*v/span) for k, v in sorted(conc.items())) + '.',
+         '', 'Idle gaps: %d, %.1f ms in all; the longest %s ms; gaps over 1 ms: %d (%.1f ms).' % (
+             len(gaps), ms(sum(gaps)), ', '.join('%.2f' % ms(g) for g in sorted(gaps)[::-1][:5]),
+             sum(g > 1e6 for g in gaps), ms(sum(g for g in gaps if g > 1e6))),
+         '', '| bin | from ms | busy | evaluation running |', '|---|---|---|---|']
+    for i in range(nb):
+        w = (t0 + (i + 1)*span//nb) - (t0 + i*span//nb)
+        L.append('| %d | %.0f | %.0f%% | %.0f%% |' % (i, ms(i*span//nb), 100*bb[i]/w, 100*be[i]/w))
+    L += ['', 'Device time by graph in the last tenth of the window: ' +
+          ', '.join('%s %.1f ms' % (g, ms(v)) for g, v in tail.most_common(8)) + '.']
+    with open(out, 'w') as f:
+        f.write('\n'.join(L) + '\n')
+    print('%d dispatches, %.1f ms window -> %s' % (len(ks), ms(span), out))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
     c = sub.add_parser('counters'); c.add_argument('out'); c.add_argument('files', nargs='+'); c.add_argument('--match', default='.')
     s = sub.add_parser('stats'); s.add_argument('out'); s.add_argument('file'); s.add_argument('--top', type=int, default=40)
     s.add_argument('--parts', action='store_true')
+    t = sub.add_parser('timeline'); t.add_argument('out'); t.add_argument('file'); t.add_argument('--last', type=float, default=None)
+    t.add_argument('--bins', type=int, default=20)
     a = ap.parse_args(argv)
     if a.cmd == 'counters':
         counters(a.out, a.files, a.match)
+    elif a.cmd == 'timeline':
+        timeline(a.out, a.file, a.last, a.bins)
     else:
         stats(a.out, a.file, a.top, a.parts)
 
