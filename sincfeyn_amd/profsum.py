@@ -109,8 +109,17 @@ def timeline(out, fn, last=None, bins=20):
     ks = [(max(s, t0), e, n) for s, e, n in ks if e > t0]
     span = t_end - t0
     # sweep over start and end points: the number of kernels running, and of evaluation kernels among them
-    ev = sorted([(s, 1, EVAL.search(n) is not None) for s, e, n in ks] + [(e, -1, EVAL.search(n) is not None) for s, e, n in ks])
+    pidx = {}
+
+    def part(n):
+        if n not in pidx:
+            pidx[n] = next(i for i, (p, rx) in enumerate(PARTS) if rx.search(n))
+        return pidx[n]
+    ev = sorted([(s, 1, EVAL.search(n) is not None, part(n)) for s, e, n in ks] +
+                [(e, -1, EVAL.search(n) is not None, part(n)) for s, e, n in ks])
     run = run_ev = 0
+    prun = [0]*len(PARTS)
+    other = [0.0]*len(PARTS)         # time with no evaluation running, shared among the parts running
     busy = ev_busy = 0
     conc = collections.Counter()
     gaps = []
@@ -123,6 +132,10 @@ def timeline(out, fn, last=None, bins=20):
         if b <= a:
             return
         conc[min(n, 4)] += b - a
+        if n and not n_ev:
+            for i, c in enumerate(prun):
+                if c:
+                    other[i] += (b - a)*c/n
         if n:
             busy += b - a
         if n_ev:
@@ -138,12 +151,13 @@ def timeline(out, fn, last=None, bins=20):
                 be[i] += c - a
             a = c
             i = min(nb - 1, i + 1)
-    for t, d, is_ev in ev:
+    for t, d, is_ev, pi in ev:
         if run == 0 and t > prev:
             gaps.append(t - prev)
         add(prev, t, run, run_ev)
         run += d
         run_ev += d if is_ev else 0
+        prun[pi] += d
         prev = t
     # the graphs of the kernels in the last tenth of the window (device time by graph)
     tail0 = t_end - span//10
@@ -159,6 +173,8 @@ def timeline(out, fn, last=None, bins=20):
          '| an integrand evaluation running (k_eval_list, k_eval, k_dd, k_qd) | %.1f | %.1f%% |' % (ms(ev_busy), 100*ev_busy/span),
          '| only other kernels running | %.1f | %.1f%% |' % (ms(busy - ev_busy), 100*(busy - ev_busy)/span),
          '| no kernel running | %.1f | %.1f%% |' % (ms(span - busy), 100*(span - busy)/span),
+         '', 'With no evaluation running, the kernels that run (time shared among those running at once): ' +
+         ', '.join('%s %.1f ms' % (PARTS[i][0], ms(v)) for i, v in sorted(enumerate(other), key=lambda x: -x[1]) if v) + '.',
          '', 'Kernels running at once (share of the window): ' +
          ', '.join('%s: %.1f%%' % ('4+' if k == 4 else k, 100*v/span) for k, v in sorted(conc.items())) + '.',
          '', 'Idle gaps: %d, %.1f ms in all; the longest %s ms; gaps over 1 ms: %d (%.1f ms).' % (
