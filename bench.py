@@ -238,6 +238,7 @@ def main():
               '(--gpus %d is not supported)' % a.gpus, file=sys.stderr)
         sys.exit(2)
     from sincfeyn_amd import sfx as S
+    S.hw_queues(a.workers)
     info = S.init(0)
     opc = json.load(open(os.path.join(ROOT, 'sincfeyn_amd', '_gen', 'opcounts.json')))
     if a.copy_probe:

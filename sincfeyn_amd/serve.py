@@ -196,6 +196,7 @@ def main():
     ap.add_argument('--check', default=None, help='folder of the reference\'s "#R" result records '
                     '(tests/golden/fixtures): every job is checked against its record')
     a = ap.parse_args()
+    sfx.hw_queues(a.workers)
     sfx.init(0)
     sfx.engine_start(a.workers, 0)
     graphs = sfx.select(*a.graphs.split(','))

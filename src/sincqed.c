@@ -9,6 +9,7 @@
  * to the library's job engine (SINCQED_WORKERS workers, default 16) and reports them in order as they
  * finish, with the results and lines of a run one after another; SINCQED_BATCH=0 runs them one at a time.
  */
+#define _POSIX_C_SOURCE 200809L      /* setenv */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -252,6 +253,14 @@ int main(int argc, char **argv){
             if(!dup) sel[ns++] = i;
         }
         if(!hit) fprintf(stderr, "# no graph matches %s in this build\n", argv[b]);
+    }
+    if(ns > 1 && envi("SINCQED_BATCH", 1)){
+        /* one hardware queue per engine worker's stream, at most 16 (HIP's default of four makes the workers'
+           streams wait for one another in queue order); read by the runtime at its first call, so set before
+           sfx_init, and a value already in the environment is kept */
+        int w = envi("SINCQED_WORKERS", 16), q = w < 4 ? 4 : w > 16 ? 16 : w;
+        char qs[8]; snprintf(qs, sizeof qs, "%d", q);
+        setenv("GPU_MAX_HW_QUEUES", qs, 0);
     }
     int rc = sfx_init(envi("SINCQED_GPU", 0));
     if(rc){ fprintf(stderr, "sincqed: %s\n", sfx_last_error()); return 3; }
