@@ -229,10 +229,14 @@ def main():
                     help='first run the library\'s plain copy kernel over this many bytes (counter calibration)')
     ap.add_argument('--batched-only', action='store_true',
                     help='only the warmup and timed batched steps (for a kernel trace of the engine\'s timeline)')
+    ap.add_argument('--grow-pipeline', type=int, default=None,
+                    help='the grow\'s frontier loop of every job (sfx_opts_t.grow_pipeline; default: the library\'s)')
     ap.add_argument('--graph-capture', type=int, default=0,
                     help='1: every refinement level\'s block chain runs as one captured HIP graph (sfx_opts_t.graph_capture)')
     a = ap.parse_args()
     JOB_KW['graph_capture'] = a.graph_capture
+    if a.grow_pipeline is not None:
+        JOB_KW['grow_pipeline'] = a.grow_pipeline
     if a.gpus != 1:
         print('bench.py: this build measures one GPU; the Sinc sums are not split across GPUs here '
               '(--gpus %d is not supported)' % a.gpus, file=sys.stderr)

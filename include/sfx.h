@@ -94,6 +94,14 @@ typedef struct {
                              graph, instantiated and launched once; ignored while checkpoints or the per-kernel
                              timers are on (both need the host between blocks).  Same results; 0 (default)
                              launches the kernels directly. */
+    int grow_pipeline;    /* the grow's frontier loop (the library's default 1).  0: each frontier's values come
+                             back to the host, whose serial sum of the reference scale A gives the keep threshold
+                             before the next launches (two host round trips per frontier).  2: the device takes
+                             the keep decisions from rigorous bounds of that threshold while the host sums and
+                             checks each frontier one behind (one round trip); a value between the bounds, or a
+                             check that fails, runs the grow again by loop 0 -- the same decisions and results
+                             either way.  1: 2 for the frontiers of at least 8192 points, 0 for the smaller ones.
+                             3: as 2 with every value between the bounds (a test of the takeover). */
 } sfx_opts_t;
 
 typedef struct {
@@ -120,6 +128,7 @@ typedef struct {
     long abmiss;          /* with opts->abcheck: skip-class points that needed the fallback (ABMISS) */
     long shell_n;         /* the grow's shell: evaluated points below the running threshold, never expanded; */
     double shell_sum, shell_abs;   /* their signed sum, and the grow's dropped |.| weight (sum.c:619-626) */
+    int grow_retries;     /* times the grow's pipelined loop handed over to the exact one (opts grow_pipeline) */
 } sfx_result_t;
 
 /* ---- library ---- */

@@ -155,6 +155,10 @@ struct Engine {
     std::vector<void*> ctx;
     EngineBackend be{};
     bool stop = false; long next = 1; int running = 0;
+    // running + queued jobs, kept under mu and read without it (a job's grow takes the pipelined frontier loop
+    // only while it has the device to itself)
+    std::atomic<int> load{0};
+    void note_load(){ load.store(running + (int)queue.size(), std::memory_order_relaxed); }
     double wait_deadline_s = 3600;            // sfx_job_wait; <= 0: no deadline
     ~Engine(){
         // at process exit: wake and join idle workers (a joinable std::thread must not be destroyed); the
@@ -179,7 +183,7 @@ inline void engine_worker(int w){
             long id = E.queue.front(); E.queue.pop_front();
             auto it = E.jobs.find(id);
             if(it == E.jobs.end()) continue;
-            j = it->second; j->state = SFX_JOB_RUNNING; j->t_start = engine_now(); E.running++;
+            j = it->second; j->state = SFX_JOB_RUNNING; j->t_start = engine_now(); E.running++; E.note_load();
         }
         // the result and the message are written here, outside the lock, and read by waiters only after
         // the state below says the job has finished
@@ -192,7 +196,7 @@ inline void engine_worker(int w){
         {
             std::lock_guard<std::mutex> lk(E.mu);
             j->rc = rc; j->t_end = engine_now();
-            j->state = rc ? SFX_JOB_FAILED : SFX_JOB_DONE; E.running--;
+            j->state = rc ? SFX_JOB_FAILED : SFX_JOB_DONE; E.running--; E.note_load();
         }
         E.done.notify_all();
     }
@@ -239,13 +243,13 @@ inline void engine_shutdown(){
             snprintf(j.err, sizeof j.err, "job %ld (%s h=%g eps=%g): the engine stopped before it started", j.id, j.name, j.h, j.eps);
             j.prog.set(STAGE_DONE);
         }
-        E.queue.clear();
+        E.queue.clear(); E.note_load();
         ws.swap(E.workers);
     }
     E.cv.notify_all(); E.done.notify_all();
     for(std::thread &t : ws) t.join();
     std::vector<void*> cs;
-    { std::lock_guard<std::mutex> lk(E.mu); cs.swap(E.ctx); E.running = 0; }
+    { std::lock_guard<std::mutex> lk(E.mu); cs.swap(E.ctx); E.running = 0; E.note_load(); }
     for(void *c : cs) E.be.ctx_destroy(c);
     { std::lock_guard<std::mutex> lb(budget().mu); budget().cap = 0; }   // no budget outside the engine
     budget().cv.notify_all();
@@ -264,7 +268,7 @@ inline int engine_submit(const EJob &proto, long *job, double hbm_bytes){
         j->id = id; j->graph = proto.graph; j->h = proto.h; j->N = proto.N; j->eps = proto.eps; j->top = proto.top;
         j->o = proto.o; memcpy(j->name, proto.name, sizeof j->name); j->t_submit = engine_now();
         E.jobs[id] = j;
-        E.queue.push_back(id);
+        E.queue.push_back(id); E.note_load();
         *job = id;
     }
     E.cv.notify_one();

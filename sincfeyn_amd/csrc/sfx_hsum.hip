@@ -66,9 +66,10 @@ struct PinBuf {
     template<class T> T *as() const { return (T*)p; }
 };
 struct HostBufs {
-    PinBuf vfe, idx, misc, rb, up;       // rb, up: staging of the small read-backs and uploads
-    void release(){ vfe.release(); idx.release(); misc.release(); rb.release(); up.release(); }
-    void trim(size_t keep){ PinBuf *all[] = {&vfe, &idx, &misc, &rb, &up}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
+    PinBuf vfe, vfe2, idx, misc, rb, up;  // vfe2: the other half of the pipelined grow's double buffer; rb, up:
+                                          // staging of the small read-backs and uploads
+    void release(){ vfe.release(); vfe2.release(); idx.release(); misc.release(); rb.release(); up.release(); }
+    void trim(size_t keep){ PinBuf *all[] = {&vfe, &vfe2, &idx, &misc, &rb, &up}; for(PinBuf *b : all) if(b->cap > keep) b->release(); }
 };
 struct Ctx {
     hipStream_t s = nullptr; JobBufs B; HostBufs H; hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -157,13 +158,15 @@ struct NodeOut { const double *v, *e2; const uint8_t *fl; };
 // nodes already on the device (dpts); the results stay in B.vfe (node_v, node_fl) and come back to the
 // pinned staging H.vfe in one copy (with st > 0 the flag bytes also carry the outer-shell class of each
 // node at that step); the caller sums what it needs in node order
-static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeOut *out){
-    if(n <= 0) return SFX_OK;
-    JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; Ctx &X = *J.ctx; int rc;
+// the launches of a node list's evaluation: the FP64 kernel, then the fallback recompute of the flagged points;
+// the results stay in B.vfe (node_v, node_e2, node_fl).  With the timers on, the three events of the context
+// bracket the two kernels.
+static int eval_nodes_launch(Job &J, const sfx_cell_t *dpts, long n, int st){
+    JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; int rc;
     // at least 4 MiB each, so a job's growing frontiers rarely reallocate
     const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
     if((rc = B.vfe.reserve(want)) || (rc = B.ddidx.reserve(std::max<size_t>(n*sizeof(uint32_t), (size_t)1 << 20))) ||
-       (rc = B.ddcnt.reserve(64)) || (rc = H.vfe.reserve(want))) return rc;
+       (rc = B.ddcnt.reserve(64))) return rc;
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
@@ -174,14 +177,26 @@ static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeO
     SFX_HIP((J.o.qd_fallback ? J.g->qd : J.g->dd)(s, 0, dpts, 0, B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>(), n, &J.P,
                     node_v(B), node_e2(B, n), node_fl(B, n)));
     if(pon) SFX_HIP(hipEventRecord(X.ev[2], s));
+    return SFX_OK;
+}
+// the timers' entries of a node list evaluated by eval_nodes_launch (its events complete; fl: its flag bytes)
+static void eval_nodes_prof(Job &J, long n, const uint8_t *fl){
+    Ctx &X = *J.ctx;
+    long m = 0; for(long i = 0; i < n; i++) m += (fl[i] & SFX_FL_DD) != 0;
+    prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m);
+}
+// nodes already on the device (dpts), evaluated; values, round terms and flag bytes back in pinned memory (*out)
+static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeOut *out){
+    if(n <= 0) return SFX_OK;
+    JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; int rc;
+    const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
+    if((rc = eval_nodes_launch(J, dpts, n, st)) || (rc = H.vfe.reserve(want))) return rc;
+    hipStream_t s = J.s;
     SFX_HIP(hipMemcpyAsync(H.vfe.p, B.vfe.p, bytes, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
     const double *hv = H.vfe.as<double>();
     out->v = hv; out->e2 = hv + n; out->fl = (const uint8_t*)(hv + 2*n);
-    if(pon){
-        long m = 0; for(long i = 0; i < n; i++) m += (out->fl[i] & SFX_FL_DD) != 0;
-        prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m);
-    }
+    if(prof().on) eval_nodes_prof(J, n, out->fl);
     return SFX_OK;
 }
 
@@ -218,6 +233,7 @@ struct GrowOut {
     double t_loop = 0, t_final = 0, t_tiers = 0, t_dilate = 0; long frontiers = 0, tier_steps = 0;
     double t_f_eval = 0, t_f_host = 0, t_f_dev = 0;   // the frontier loop: evaluation + copy back, host sums, the rest
     double gshs = 0; long gshn = 0;   // the shell: evaluated points below the running threshold, never expanded
+    long retries = 0;                 // the pipelined frontier loop's hand-overs to the exact one
     long nskip = 0, nmiss = 0;        // frontier points of the skip class, and those that needed the fallback
 };
 
@@ -558,7 +574,7 @@ static int hsum_core(Ctx &X, int graph, double h, int N, double eps, int top, co
         res->abskipped += G.nskip; res->abmiss += G.nmiss;
         res->nev_level[top] = G.nev;
         res->ref = ref = G.ref; res->prune = G.drop;
-        res->shell_n = G.gshn; res->shell_sum = G.gshs; res->shell_abs = G.drop;
+        res->shell_n = G.gshn; res->shell_sum = G.gshs; res->shell_abs = G.drop; res->grow_retries = (int)G.retries;
         // top = 0 (sum.c:619-626): every evaluated point is in the sum; the prune column is the signed sum of
         // the never-expanded shell, an estimate of what lies beyond it
         if(!J.o.box_scan && top == 0 && G.gshn) res->prune = fabs(G.gshs);

@@ -106,7 +106,7 @@ def serve(stream, results=None, progress=False, wait_timeout=600.0):
         qs.append(tq)
         ev += r['nev']
         if results is not None:
-            results[i] = r
+            results[i] = dict(r, queue_s=tq, run_s=tr)
     th.join()
     wall = time.perf_counter() - t0
     return {'jobs': len(stream), 'points': ev, 'wall_s': wall, 'points_per_s': ev/wall,
@@ -160,7 +160,9 @@ def sweep(rates, seed, graphs, names, refs=None, jobs_for=None, progress=False):
         jobs, worst, bad, checked = [], 0.0, 0, 0
         for (a, g, h, N, eps, top), r in zip(stream, res):
             j = {'graph': names[g], 'h': h, 'N': N, 'eps': eps, 'top': top, 'arrival_s': round(a, 4),
-                 'S_h': r['S'][0], 'nev': r['nev'], 'nq': r['nq'], 'kept': r['kept_top']}
+                 'S_h': r['S'][0], 'nev': r['nev'], 'nq': r['nq'], 'kept': r['kept_top'],
+                 'queue_s': round(r['queue_s'], 5), 'run_s': round(r['run_s'], 5),
+                 'coarsest_s': round(r['t_grow'], 5), 'dilation_s': round(r['t_dilate'], 5), 'refinement_s': round(r['t_refine'], 5)}
             rec = refs.get((names[g], h, N, eps)) if refs else None
             if rec is not None:
                 ok, d = check_job(r, rec)

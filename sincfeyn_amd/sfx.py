@@ -33,7 +33,7 @@ class Opts(C.Structure):
                 ('block_points', C.c_long), ('verbose', C.c_int), ('ckpt_every', C.c_double),
                 ('ckpt_dir', C.c_char*256), ('ckpt_stop_after', C.c_int), ('qd_fallback', C.c_int),
                 ('box_scan', C.c_int), ('abcheck', C.c_int),
-                ('growskip', C.c_int), ('graph_capture', C.c_int)]
+                ('growskip', C.c_int), ('graph_capture', C.c_int), ('grow_pipeline', C.c_int)]
 
 
 class Result(C.Structure):
@@ -45,7 +45,7 @@ class Result(C.Structure):
                 ('t_grow', C.c_double), ('t_dilate', C.c_double), ('t_refine', C.c_double),
                 ('resumed', C.c_int), ('resume_lev', C.c_int), ('resume_next', C.c_long), ('resume_ncells', C.c_long),
                 ('ckpts_written', C.c_int), ('abmiss', C.c_long),
-                ('shell_n', C.c_long), ('shell_sum', C.c_double), ('shell_abs', C.c_double)]
+                ('shell_n', C.c_long), ('shell_sum', C.c_double), ('shell_abs', C.c_double), ('grow_retries', C.c_int)]
 
 
 class GraphInfo(C.Structure):
