@@ -442,7 +442,7 @@ extern "C" const char *sfx_info(void){ return lib().init ? lib().info.c_str() : 
 extern "C" void sfx_opts_default(sfx_opts_t *o){
     memset(o, 0, sizeof *o);
     o->abserr0 = 1e-15; o->abrel = 1e-3; o->nearf = 1e-3; o->ddslack = 1; o->abskip = 1; o->dropunres = 1;
-    o->keepbt = 0; o->btmargin = 2; o->block_points = 0; o->verbose = 0; o->growskip = 2;
+    o->keepbt = 0; o->btmargin = 2; o->block_points = 0; o->verbose = 0; o->growskip = 2; o->grow_pipeline = 1;
 }
 
 extern "C" int sfx_graph_count(void){ return SFX_NGRAPHS; }

@@ -80,3 +80,13 @@ def test_cli_list_and_clean_failure_without_gpu():
     assert 'g3_lbl' in p.stdout
     p = subprocess.run([cli, 'bogus'], capture_output=True, text=True, timeout=60)
     assert p.returncode == 1
+
+
+def test_run_option_defaults(S):
+    """sfx_opts_default: the reference's environment defaults (main.c:176-185, sum.c:558-592) and the
+    library's own (the pipelined grow steps on, graph capture off)"""
+    o = S.opts()
+    assert (o.abserr0, o.abrel, o.nearf, o.ddslack) == (1e-15, 1e-3, 1e-3, 1.0)
+    assert (o.abskip, o.dropunres, o.keepbt, o.btmargin, o.growskip) == (1, 1, 0, 2, 2)
+    assert (o.abcheck, o.box_scan, o.qd_fallback, o.graph_capture) == (0, 0, 0, 0)
+    assert o.grow_pipeline == 1

@@ -51,6 +51,28 @@ def test_pipelined_grow_equals_exact_loop(sfx, name, h, N, eps, top):
     _same(fo, ex)
 
 
+# the run options that change the grow's decisions or bookkeeping: ABCHECK (the skip class tested in full),
+# no |.|-sum skipping, every boundary table kept (sumkey.h:446-453), no skip generations at top = 0
+OPTION_CASES = [('v3_03', 1.0, 97, 1e-12, None, {'abcheck': 1}), ('g3_07', 1.5, 33, 1e-12, None, {'abskip': 0}),
+                ('g2_00', 1.0, 97, 1e-12, None, {'keepbt': 1}), ('v2_00', 1.0, 97, 1e-12, 0, {'growskip': 0}),
+                ('g2_03', 1.0, 97, 1e-12, 0, {'abcheck': 1})]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('name,h,N,eps,top,kw', OPTION_CASES)
+def test_pipelined_grow_with_run_options(sfx, name, h, N, eps, top, kw):
+    top = sfx.default_top(h) if top is None else top
+    ex = sfx.hsum(name, h, N, eps, top, grow_pipeline=0, **kw)
+    pl = sfx.hsum(name, h, N, eps, top, grow_pipeline=2, **kw)
+    fo = sfx.hsum(name, h, N, eps, top, grow_pipeline=3, **kw)
+    print(name, h, N, eps, top, kw, 'nev %d kept %d abmiss %d retries %d / %d' % (
+        pl['nev'], pl['kept_top'], pl['abmiss'], pl['grow_retries'], fo['grow_retries']), flush=True)
+    assert fo['grow_retries'] == 1 and pl['grow_retries'] in (0, 1)
+    for r in (pl, fo):
+        _same(r, ex)
+        assert r['abmiss'] == ex['abmiss']
+
+
 @pytest.mark.gpu
 def test_pipelined_grow_in_the_engine(sfx):
     """the same jobs through the engine's workers (their own streams and buffers) as one after another"""
