@@ -100,8 +100,10 @@ typedef struct {
                              the keep decisions from rigorous bounds of that threshold while the host sums and
                              checks each frontier one behind (one round trip); a value between the bounds, or a
                              check that fails, runs the grow again by loop 0 -- the same decisions and results
-                             either way.  1: 2 for the frontiers of at least 8192 points, 0 for the smaller ones.
-                             3: as 2 with every value between the bounds (a test of the takeover). */
+                             either way.  1: 2 for the frontiers of at least 8192 points, 0 for the smaller ones,
+                             and 0 throughout while other engine jobs run or wait (the device is shared: the exact
+                             loop costs it less).  3: as 2 with every value between the bounds (a test of the
+                             takeover). */
 } sfx_opts_t;
 
 typedef struct {
