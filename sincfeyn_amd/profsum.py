@@ -6,6 +6,10 @@
     python -m sincfeyn_amd.profsum stats OUT.md KERNEL_STATS_CSV [--top N] [--parts]
         the first N rows of a kernel-stats CSV (rocprofv3 --kernel-trace --stats) as a markdown table; with
         --parts, short kernel names and first a table of the device time by part of the hierarchical sum
+    python -m sincfeyn_amd.profsum valu OUT.json COUNTER_CSV... --profile KERNELS_JSON [--kernel NAME]
+        the FP64 and VALU issue figures of every level kernel from two counter passes, and of NAME (default
+        k_eval_list<g3_lbl, 0>) also the executed FP64 operations and instructions per point: the table bench.py
+        reads for its roofline's executed rate
     python -m sincfeyn_amd.profsum timeline OUT.md KERNEL_TRACE_CSV [--last SECONDS] [--bins N]
         the device's timeline from a kernel trace (rocprofv3 --kernel-trace): over the last SECONDS of the
         trace (default all of it), the time some kernel runs, the time an integrand evaluation runs, the
@@ -92,6 +96,55 @@ def stats(out, fn, top, parts=False):
     with open(out, 'w') as f:
         f.write('\n'.join(lines) + '\n')
     print('%d rows -> %s' % (min(top, len(rows)), out))
+
+
+def valu(out, files, profile, kernel):
+    """the FP64 issue figures of the level kernels from the two counter passes (SQ_INSTS_VALU_{ADD,MUL,FMA,
+    TRANS}_F64; SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, SQ_BUSY_CYCLES, GRBM_GUI_ACTIVE), per dispatch: a wave64
+    VALU instruction occupies a 16-lane SIMD for 4 cycles, GRBM_GUI_ACTIVE is summed over the 8 XCDs, the
+    device has 1024 SIMDs; with the bench's per-kernel table (bench.py --profile-out) the valid points per
+    dispatch of `kernel` (a short name, k_eval_list<g3_lbl, 0>) and its instructions per point"""
+    acc = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                n = short_name(row.get('Kernel_Name', ''))
+                if not n.startswith('k_eval_list<'):
+                    continue
+                c = row['Counter_Name']
+                acc[n][c] += float(row['Counter_Value'])
+                disp[n][c].add(row.get('Dispatch_Id') or row.get('Correlation_Id'))
+    per = {n: {c: v/len(disp[n][c]) for c, v in cs.items()} for n, cs in acc.items()}
+
+    def figures(c):
+        fp = sum(c.get('SQ_INSTS_VALU_%s_F64' % k, 0.0) for k in ('ADD', 'MUL', 'FMA', 'TRANS'))
+        cyc = 1024*c['GRBM_GUI_ACTIVE']/8
+        return {'fp64_share_of_valu': round(fp/c['SQ_INSTS_VALU'], 4), 'valu_issue_busy': round(4*c['SQ_INSTS_VALU']/cyc, 4),
+                'fp64_issue_busy': round(4*fp/cyc, 4), 'fma_share_of_fp64': round(c.get('SQ_INSTS_VALU_FMA_F64', 0.0)/fp, 4)}
+    need = ('SQ_INSTS_VALU', 'GRBM_GUI_ACTIVE', 'SQ_INSTS_VALU_ADD_F64', 'SQ_INSTS_VALU_MUL_F64')
+    table = {n: dict(figures(c), dispatches=len(disp[n]['SQ_INSTS_VALU'])) for n, c in sorted(per.items())
+             if all(k in c for k in need) and c['GRBM_GUI_ACTIVE'] > 0 and
+             sum(c.get('SQ_INSTS_VALU_%s_F64' % k, 0.0) for k in ('ADD', 'MUL', 'FMA', 'TRANS')) > 0}
+    c = per[kernel]
+    fp = sum(c['SQ_INSTS_VALU_%s_F64' % k] for k in ('ADD', 'MUL', 'FMA', 'TRANS'))
+    g, ab = re.match(r'k_eval_list<(\w+), (\d)>', kernel).groups()
+    ent = [e for e in json.load(open(profile)) if e['graph'] == g and e['kind'] == ('level_eval_skip' if ab == '0' else 'level_eval')]
+    pts = sum(e['points'] for e in ent)/sum(e['launches'] for e in ent)
+    res = {'kernel': 'k_eval_list<%s, AB=%s>' % (g, ab), 'dispatches': table[kernel]['dispatches'],
+           'valid_points_per_dispatch': pts, 'per_dispatch': c,
+           'executed_fp64_flops_per_dispatch': 64*(c['SQ_INSTS_VALU_ADD_F64'] + c['SQ_INSTS_VALU_MUL_F64'] +
+                                                   c.get('SQ_INSTS_VALU_TRANS_F64', 0.0) + 2*c.get('SQ_INSTS_VALU_FMA_F64', 0.0)),
+           'fp64_instructions_per_point': round(64*fp/pts, 1), 'valu_instructions_per_point': round(64*c['SQ_INSTS_VALU']/pts, 1)}
+    res.update(figures(c))
+    res['note'] = ('executed FP64 operations = 64 lanes x (ADD + MUL + TRANS + 2 FMA) per wave instruction; issue busy = '
+                   'wave instructions x 4 cycles (a wave64 VALU instruction on a 16-lane SIMD) / (1024 SIMDs x '
+                   'GRBM_GUI_ACTIVE/8 cycles, the counter summed over 8 XCDs)')
+    res['all_level_kernels'] = table
+    with open(out, 'w') as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print('%d level kernels -> %s; %s: FP64 issue %.3f, VALU issue %.3f' % (len(table), out, kernel, res['fp64_issue_busy'],
+                                                                              res['valu_issue_busy']))
 
 
 EVAL = re.compile(r'^(k_eval_list<|k_eval<|k_dd<|k_qd<)')
@@ -197,11 +250,15 @@ def main(argv=None):
     c = sub.add_parser('counters'); c.add_argument('out'); c.add_argument('files', nargs='+'); c.add_argument('--match', default='.')
     s = sub.add_parser('stats'); s.add_argument('out'); s.add_argument('file'); s.add_argument('--top', type=int, default=40)
     s.add_argument('--parts', action='store_true')
+    v = sub.add_parser('valu'); v.add_argument('out'); v.add_argument('files', nargs='+'); v.add_argument('--profile', required=True)
+    v.add_argument('--kernel', default='k_eval_list<g3_lbl, 0>')
     t = sub.add_parser('timeline'); t.add_argument('out'); t.add_argument('file'); t.add_argument('--last', type=float, default=None)
     t.add_argument('--bins', type=int, default=20)
     a = ap.parse_args(argv)
     if a.cmd == 'counters':
         counters(a.out, a.files, a.match)
+    elif a.cmd == 'valu':
+        valu(a.out, a.files, a.profile, a.kernel)
     elif a.cmd == 'timeline':
         timeline(a.out, a.file, a.last, a.bins)
     else:
