@@ -10,6 +10,10 @@
         the FP64 and VALU issue figures of every level kernel from two counter passes, and of NAME (default
         k_eval_list<g3_lbl, 0>) also the executed FP64 operations and instructions per point: the table bench.py
         reads for its roofline's executed rate
+    python -m sincfeyn_amd.profsum traffic OUT.json FETCH_CSV WRITE_CSV --profile KERNELS_JSON [--kernel NAME]
+            [--copy-bytes N] [--bytes-per-point B]
+        HBM bytes per dispatch of NAME from a FETCH_SIZE and a WRITE_SIZE pass, each counter scaled by what it
+        reads for the library's copy kernel of N bytes (bench.py --copy-probe N), beside the algorithmic bytes
     python -m sincfeyn_amd.profsum timeline OUT.md KERNEL_TRACE_CSV [--last SECONDS] [--bins N]
         the device's timeline from a kernel trace (rocprofv3 --kernel-trace): over the last SECONDS of the
         trace (default all of it), the time some kernel runs, the time an integrand evaluation runs, the
@@ -147,6 +151,40 @@ def valu(out, files, profile, kernel):
                                                                               res['valu_issue_busy']))
 
 
+def traffic(out, files, profile, kernel, copy_bytes, bytes_per_point):
+    """HBM bytes per dispatch of `kernel` from a FETCH_SIZE and a WRITE_SIZE pass (KiB per dispatch), each
+    scaled by what the counter reads for the library's copy kernel of copy_bytes (bench.py --copy-probe); beside
+    them the algorithmic bytes, bytes_per_point x the valid points per dispatch of the bench's per-kernel table"""
+    acc = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(lambda: collections.defaultdict(set))
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                n = short_name(row.get('Kernel_Name', ''))
+                c = row['Counter_Name']
+                if c not in ('FETCH_SIZE', 'WRITE_SIZE') or not (n == kernel or 'copy_probe' in n):
+                    continue
+                n = 'copy' if 'copy_probe' in n else n
+                acc[n][c] += float(row['Counter_Value'])
+                disp[n][c].add(row.get('Dispatch_Id') or row.get('Correlation_Id'))
+    m = {n: {c: v/len(disp[n][c]) for c, v in cs.items()} for n, cs in acc.items()}
+    ff = copy_bytes/1024/m['copy']['FETCH_SIZE']; wf = copy_bytes/1024/m['copy']['WRITE_SIZE']
+    g, ab = re.match(r'k_eval_list<(\w+), (\d)>', kernel).groups()
+    ent = [e for e in json.load(open(profile)) if e['graph'] == g and e['kind'] == ('level_eval_skip' if ab == '0' else 'level_eval')]
+    pts = sum(e['points'] for e in ent)/sum(e['launches'] for e in ent)
+    k = m[kernel]
+    res = {'kernel': 'k_eval_list<%s, AB=%s>' % (g, ab), 'dispatches': len(disp[kernel]['FETCH_SIZE']),
+           'copy_probe': {'bytes_read': copy_bytes, 'bytes_written': copy_bytes, 'FETCH_SIZE_kib': m['copy']['FETCH_SIZE'],
+                          'WRITE_SIZE_kib': m['copy']['WRITE_SIZE'], 'fetch_factor': ff, 'write_factor': wf},
+           'FETCH_SIZE_kib_per_dispatch': k['FETCH_SIZE'], 'WRITE_SIZE_kib_per_dispatch': k['WRITE_SIZE'],
+           'traffic_bytes_per_dispatch': (k['FETCH_SIZE']*ff + k['WRITE_SIZE']*wf)*1024,
+           'algorithmic_bytes_per_dispatch': bytes_per_point*pts, 'valid_points_per_dispatch': pts}
+    with open(out, 'w') as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+    print('%s: %.4g bytes per dispatch (algorithmic %.4g), fetch factor %.3f, write factor %.3f -> %s' % (
+        kernel, res['traffic_bytes_per_dispatch'], res['algorithmic_bytes_per_dispatch'], ff, wf, out))
+
+
 EVAL = re.compile(r'^(k_eval_list<|k_eval<|k_dd<|k_qd<)')
 
 
@@ -252,11 +290,16 @@ def main(argv=None):
     s.add_argument('--parts', action='store_true')
     v = sub.add_parser('valu'); v.add_argument('out'); v.add_argument('files', nargs='+'); v.add_argument('--profile', required=True)
     v.add_argument('--kernel', default='k_eval_list<g3_lbl, 0>')
+    r = sub.add_parser('traffic'); r.add_argument('out'); r.add_argument('files', nargs='+'); r.add_argument('--profile', required=True)
+    r.add_argument('--kernel', default='k_eval_list<g3_lbl, 0>'); r.add_argument('--copy-bytes', type=int, default=1 << 30)
+    r.add_argument('--bytes-per-point', type=float, default=9 + 20/128)
     t = sub.add_parser('timeline'); t.add_argument('out'); t.add_argument('file'); t.add_argument('--last', type=float, default=None)
     t.add_argument('--bins', type=int, default=20)
     a = ap.parse_args(argv)
     if a.cmd == 'counters':
         counters(a.out, a.files, a.match)
+    elif a.cmd == 'traffic':
+        traffic(a.out, a.files, a.profile, a.kernel, a.copy_bytes, a.bytes_per_point)
     elif a.cmd == 'valu':
         valu(a.out, a.files, a.profile, a.kernel)
     elif a.cmd == 'timeline':
