@@ -160,8 +160,9 @@ struct NodeOut { const double *v, *e2; const uint8_t *fl; };
 // node at that step); the caller sums what it needs in node order
 // the launches of a node list's evaluation: the FP64 kernel, then the fallback recompute of the flagged points;
 // the results stay in B.vfe (node_v, node_e2, node_fl).  With the timers on, the three events of the context
-// bracket the two kernels.
-static int eval_nodes_launch(Job &J, const sfx_cell_t *dpts, long n, int st){
+// bracket the two kernels.  zeroed: the fallback list's count (B.ddcnt) is already 0 (the grow's previous
+// frontier cleared it), one launch fewer.
+static int eval_nodes_launch(Job &J, const sfx_cell_t *dpts, long n, int st, bool zeroed = false){
     JobBufs &B = J.ctx->B; Ctx &X = *J.ctx; int rc;
     // at least 4 MiB each, so a job's growing frontiers rarely reallocate
     const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
@@ -170,7 +171,7 @@ static int eval_nodes_launch(Job &J, const sfx_cell_t *dpts, long n, int st){
     hipStream_t s = J.s;
     const bool pon = prof().on;
     if(pon){ if((rc = prof_events(X))) return rc; SFX_HIP(hipEventRecord(X.ev[0], s)); }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, B.ddcnt.as<uint32_t>(), 1L);
+    if(!zeroed) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, B.ddcnt.as<uint32_t>(), 1L);
     SFX_HIP(J.g->eval(s, MODE_NODES, 2, dpts, nullptr, nullptr, n, st, &J.P, node_v(B), node_fl(B, n),
                       B.ddidx.as<uint32_t>(), B.ddcnt.as<uint32_t>()));
     if(pon) SFX_HIP(hipEventRecord(X.ev[1], s));
@@ -186,11 +187,11 @@ static void eval_nodes_prof(Job &J, long n, const uint8_t *fl){
     prof_add(J.g->name, SFX_K_NODES_EVAL, prof_ms(X, 0, 1), n, 0, 0); prof_add(J.g->name, SFX_K_NODES_DD, prof_ms(X, 1, 2), m, 0, m);
 }
 // nodes already on the device (dpts), evaluated; values, round terms and flag bytes back in pinned memory (*out)
-static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeOut *out){
+static int eval_nodes_core(Job &J, const sfx_cell_t *dpts, long n, int st, NodeOut *out, bool zeroed = false){
     if(n <= 0) return SFX_OK;
     JobBufs &B = J.ctx->B; HostBufs &H = J.ctx->H; int rc;
     const size_t bytes = (size_t)n*(2*sizeof(double) + 1), want = std::max(bytes, (size_t)4 << 20);
-    if((rc = eval_nodes_launch(J, dpts, n, st)) || (rc = H.vfe.reserve(want))) return rc;
+    if((rc = eval_nodes_launch(J, dpts, n, st, zeroed)) || (rc = H.vfe.reserve(want))) return rc;
     hipStream_t s = J.s;
     SFX_HIP(hipMemcpyAsync(H.vfe.p, B.vfe.p, bytes, hipMemcpyDeviceToHost, s));
     SFX_HIP(hipStreamSynchronize(s));
